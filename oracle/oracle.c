@@ -1,0 +1,354 @@
+/*
+ * oracle.c -- CPU restatement of the zeknox LDE + Merkle-commit path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP library is
+ * compared against; it is never linked into, loaded by or called from the
+ * product library.  Only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline use it.
+ *
+ * Clarity over speed: field arithmetic uses unsigned __int128 and the textbook
+ * definitions; Poseidon uses the naive partial rounds (full 12x12 MDS every
+ * round), which is the permutation's definition.  The reference's CPU path
+ * computes the same function through the "fast partial round" factorisation.
+ *
+ * What each function restates (reference file:line) and what pins it:
+ *   gl_add/gl_sub/gl_mul     ff/goldilocks.hpp:139-177 (modulo_add/sub, reduce128)
+ *                            pinned: brute force vs Python big ints on edge values
+ *   gl_omega                 ff/gl64_params.hpp:15-49 (OMEGA[k]) from OMEGA[32]
+ *                            pinned: closed forms (order 2^k exactly, OMEGA[1]=p-1,
+ *                            OMEGA[2]=2^48) and the table's OMEGA[32]
+ *   poseidon_permute         poseidon/poseidon.cu:613-838 (naive rounds),
+ *                            poseidon/poseidon.cpp:75-110,401-408
+ *                            pinned: KATs tests/tests.cu:281-292, Go caps
+ *                            wrappers/go/lib/lib_test.go:197-213, reference CPU
+ *   poseidon2_permute        poseidon2/poseidon2.cu:65-209
+ *                            pinned: KAT tests/tests.cu:425-428, reference CPU
+ *   hash_one / hash_two      poseidon/poseidon_permutation.hpp:83-139
+ *   fill_digests             merkle/merkle.cpp:19-161 (linear forest layout)
+ *                            pinned: Go caps KATs, reference CPU (oracle/_ref)
+ *   ntt / lde                ntt/ntt.cu:106-158,253-309,477-564; kernels.cu:14-212
+ *                            (natural-order coset NTT, zero-padded LDE)
+ *                            pinned: brute-force O(n^2) DFT, closed forms,
+ *                            inverse-of-forward identity, Horner spot checks
+ *   transpose_rev            ntt/kernels.cu:94-135; wrappers/rust/tests/utils.rs:15-27
+ *                            pinned: brute-force permutation formula in Python
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../zeknox_amd/csrc/gl_constants.h"
+
+typedef uint64_t u64;
+typedef unsigned __int128 u128;
+
+#define GL_P 0xFFFFFFFF00000001ull
+#define GL_OMEGA32 1753635133440165772ull /* gl64_params.hpp:48 */
+
+/* ------------------------------------------------------------------ field */
+
+u64 orc_gl_add(u64 a, u64 b) { return (u64)(((u128)a + b) % GL_P); }
+u64 orc_gl_sub(u64 a, u64 b) { return (u64)(((u128)a + GL_P - (b % GL_P)) % GL_P); }
+u64 orc_gl_mul(u64 a, u64 b) { return (u64)(((u128)a * b) % GL_P); }
+
+u64 orc_gl_pow(u64 b, u64 e)
+{
+    u64 r = 1;
+    b %= GL_P;
+    while (e) {
+        if (e & 1) r = orc_gl_mul(r, b);
+        b = orc_gl_mul(b, b);
+        e >>= 1;
+    }
+    return r;
+}
+
+u64 orc_gl_inv(u64 a) { return orc_gl_pow(a, GL_P - 2); }
+
+/* primitive 2^k-th root of unity, OMEGA[k] of gl64_params.hpp */
+u64 orc_gl_omega(unsigned k)
+{
+    u64 w = GL_OMEGA32;
+    for (unsigned i = k; i < 32; i++) w = orc_gl_mul(w, w);
+    return w;
+}
+
+static u64 sbox7(u64 x)
+{
+    u64 x2 = orc_gl_mul(x, x);
+    u64 x4 = orc_gl_mul(x2, x2);
+    return orc_gl_mul(orc_gl_mul(x4, x2), x);
+}
+
+/* --------------------------------------------------------------- poseidon */
+
+static void poseidon_mds(u64 *s)
+{
+    u64 out[12];
+    for (int r = 0; r < 12; r++) {
+        u128 acc = 0;
+        for (int i = 0; i < 12; i++) acc += (u128)s[(i + r) % 12] * ZK_POSEIDON_MDS_CIRC[i];
+        acc += (u128)s[r] * ZK_POSEIDON_MDS_DIAG[r];
+        out[r] = (u64)(acc % GL_P);
+    }
+    memcpy(s, out, sizeof(out));
+}
+
+void orc_poseidon_permute(u64 *s)
+{
+    int rc = 0;
+    for (int r = 0; r < 4; r++, rc++) {
+        for (int i = 0; i < 12; i++) s[i] = sbox7(orc_gl_add(s[i], ZK_POSEIDON_RC[12 * rc + i]));
+        poseidon_mds(s);
+    }
+    for (int r = 0; r < 22; r++, rc++) {
+        for (int i = 0; i < 12; i++) s[i] = orc_gl_add(s[i], ZK_POSEIDON_RC[12 * rc + i]);
+        s[0] = sbox7(s[0]);
+        poseidon_mds(s);
+    }
+    for (int r = 0; r < 4; r++, rc++) {
+        for (int i = 0; i < 12; i++) s[i] = sbox7(orc_gl_add(s[i], ZK_POSEIDON_RC[12 * rc + i]));
+        poseidon_mds(s);
+    }
+}
+
+/* -------------------------------------------------------------- poseidon2 */
+
+/* M4 = [[5,7,1,3],[4,6,1,1],[1,3,5,7],[1,1,4,6]] applied to one 4-block,
+ * the matrix that poseidon2.cu:65-82 computes with additions. */
+static void p2_m4(u64 *x)
+{
+    static const u64 M4[4][4] = {{5, 7, 1, 3}, {4, 6, 1, 1}, {1, 3, 5, 7}, {1, 1, 4, 6}};
+    u64 y[4];
+    for (int r = 0; r < 4; r++) {
+        u128 acc = 0;
+        for (int c = 0; c < 4; c++) acc += (u128)M4[r][c] * x[c];
+        y[r] = (u64)(acc % GL_P);
+    }
+    memcpy(x, y, sizeof(y));
+}
+
+static void p2_external(u64 *s)
+{
+    for (int b = 0; b < 12; b += 4) p2_m4(s + b);
+    for (int c = 0; c < 4; c++) {
+        u64 sum = orc_gl_add(orc_gl_add(s[c], s[4 + c]), s[8 + c]);
+        for (int b = 0; b < 12; b += 4) s[b + c] = orc_gl_add(s[b + c], sum);
+    }
+}
+
+void orc_poseidon2_permute(u64 *s)
+{
+    p2_external(s);
+    for (int r = 0; r < 4; r++) {
+        for (int i = 0; i < 12; i++) s[i] = sbox7(orc_gl_add(s[i], ZK_POSEIDON2_RC[12 * r + i]));
+        p2_external(s);
+    }
+    for (int r = 4; r < 26; r++) {
+        s[0] = sbox7(orc_gl_add(s[0], ZK_POSEIDON2_RC[12 * r]));
+        u64 sum = 0;
+        for (int i = 0; i < 12; i++) sum = orc_gl_add(sum, s[i]);
+        for (int i = 0; i < 12; i++) s[i] = orc_gl_add(orc_gl_mul(s[i], ZK_POSEIDON2_DIAG[i]), sum);
+    }
+    for (int r = 26; r < 30; r++) {
+        for (int i = 0; i < 12; i++) s[i] = sbox7(orc_gl_add(s[i], ZK_POSEIDON2_RC[12 * r + i]));
+        p2_external(s);
+    }
+}
+
+/* ---------------------------------------------------------------- sponge */
+
+enum { HASH_POSEIDON = 0, HASH_POSEIDON2 = 3 };
+
+static void permute(u64 *s, int hash_type)
+{
+    if (hash_type == HASH_POSEIDON2) orc_poseidon2_permute(s);
+    else orc_poseidon_permute(s);
+}
+
+void orc_hash_one(const u64 *in, u64 n, u64 *out, int hash_type)
+{
+    if (n <= 4) {
+        for (u64 i = 0; i < 4; i++) out[i] = i < n ? in[i] : 0;
+        return;
+    }
+    u64 s[12] = {0};
+    for (u64 idx = 0; idx < n; idx += 8) {
+        u64 len = n - idx < 8 ? n - idx : 8;
+        memcpy(s, in + idx, len * sizeof(u64)); /* overwrite, no padding */
+        permute(s, hash_type);
+    }
+    memcpy(out, s, 4 * sizeof(u64));
+}
+
+void orc_hash_two(const u64 *l, const u64 *r, u64 *out, int hash_type)
+{
+    u64 s[12] = {0};
+    memcpy(s, l, 4 * sizeof(u64));
+    memcpy(s + 4, r, 4 * sizeof(u64));
+    permute(s, hash_type);
+    memcpy(out, s, 4 * sizeof(u64));
+}
+
+/* ----------------------------------------------------------------- merkle */
+
+/* Linear Merkle forest, merkle.cpp:19-124.  Returns 0, or -1 when the sizes
+ * are inconsistent (the reference asserts). */
+int orc_fill_digests(u64 *dig, u64 *cap, const u64 *leaves, u64 nd, u64 nc, u64 nl,
+                     u64 ls, u64 h, int hash_type)
+{
+    if (nc == nl) {
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < (long long)nl; i++) orc_hash_one(leaves + i * ls, ls, cap + 4 * i, hash_type);
+        return 0;
+    }
+    u64 sdl = nd >> h, sl = nl >> h;
+    if (sdl == 0 || sl == 0 || nd / sdl != nc || nl / sl != nc) return -1;
+    for (u64 k = 0; k < nc; k++) {
+        const u64 *lv = leaves + k * sl * ls;
+        u64 *dg = dig + k * sdl * 4;
+        u64 *cp = cap + k * 4;
+        if (sl == 1) {
+            orc_hash_one(lv, ls, dg, hash_type);
+            memcpy(cp, dg, 32);
+            continue;
+        }
+        if (sl == 2) {
+            orc_hash_one(lv, ls, dg, hash_type);
+            orc_hash_one(lv + ls, ls, dg + 4, hash_type);
+            orc_hash_two(dg, dg + 4, cp, hash_type);
+            continue;
+        }
+        u64 *lh = dg + (sdl - sl) * 4;
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < (long long)sl; i++) orc_hash_one(lv + i * ls, ls, lh + 4 * i, hash_type);
+        /* node j (heap index j+2) has children 2j+2, 2j+3; level with 2^r
+         * nodes starts at 2^r - 2 */
+        for (u64 w = sl / 2; w >= 2; w /= 2) {
+            u64 base = w - 2;
+#pragma omp parallel for schedule(static)
+            for (long long i = 0; i < (long long)w; i++) {
+                u64 j = base + i;
+                orc_hash_two(dg + 4 * (2 * j + 2), dg + 4 * (2 * j + 3), dg + 4 * j, hash_type);
+            }
+        }
+        orc_hash_two(dg, dg + 4, cp, hash_type);
+    }
+    return 0;
+}
+
+/* -------------------------------------------------------------------- ntt */
+
+static unsigned bitrev(u64 x, unsigned bits)
+{
+    u64 r = 0;
+    for (unsigned i = 0; i < bits; i++) r |= ((x >> i) & 1) << (bits - 1 - i);
+    return (unsigned)r;
+}
+
+/* In-place natural-order -> natural-order DFT of size 2^lg over one column:
+ * out[j] = sum_i in[i] * w^(i*j), w = OMEGA[lg] (or its inverse).
+ * Bit-reverse + iterative radix-2 Cooley-Tukey. */
+static void ntt_column(u64 *a, unsigned lg, int inverse)
+{
+    u64 n = 1ull << lg;
+    for (u64 i = 0; i < n; i++) {
+        u64 j = bitrev(i, lg);
+        if (j > i) { u64 t = a[i]; a[i] = a[j]; a[j] = t; }
+    }
+    for (unsigned s = 1; s <= lg; s++) {
+        u64 m = 1ull << s, half = m >> 1;
+        u64 wm = orc_gl_omega(s);
+        if (inverse) wm = orc_gl_inv(wm);
+        u64 *tw = (u64 *)malloc(half * sizeof(u64));
+        tw[0] = 1;
+        for (u64 k = 1; k < half; k++) tw[k] = orc_gl_mul(tw[k - 1], wm);
+        for (u64 base = 0; base < n; base += m)
+            for (u64 k = 0; k < half; k++) {
+                u64 u = a[base + k], v = orc_gl_mul(a[base + k + half], tw[k]);
+                a[base + k] = orc_gl_add(u, v);
+                a[base + k + half] = orc_gl_sub(u, v);
+            }
+        free(tw);
+    }
+}
+
+/* compute_batched_ntt semantics (ntt.cu:253-309 / 106-158) on host memory:
+ * forward: [x_i *= g^i if coset] then NTT, natural order;
+ * inverse: iNTT, [x_i *= g^i if coset -- the reference multiplies by g^i, not
+ * g^-i, ntt.cu:137-139], then x_i *= 2^-lg. */
+void orc_ntt(u64 *data, unsigned lg, unsigned batches, int inverse, int coset, u64 gen)
+{
+    u64 n = 1ull << lg;
+#pragma omp parallel for schedule(dynamic)
+    for (long long b = 0; b < (long long)batches; b++) {
+        u64 *a = data + b * n;
+        if (!inverse && coset) {
+            u64 g = 1;
+            for (u64 i = 0; i < n; i++) { a[i] = orc_gl_mul(a[i], g); g = orc_gl_mul(g, gen); }
+        }
+        ntt_column(a, lg, inverse);
+        if (inverse) {
+            u64 ninv = orc_gl_inv(n % GL_P), g = 1;
+            for (u64 i = 0; i < n; i++) {
+                if (coset) { a[i] = orc_gl_mul(a[i], g); g = orc_gl_mul(g, gen); }
+                a[i] = orc_gl_mul(a[i], ninv);
+            }
+        }
+    }
+}
+
+/* compute_batched_lde semantics (ntt.cu:477-564): per column zero-pad 2^lg_n
+ * coefficients to 2^(lg_n+rate), then the batched NTT above on size
+ * 2^(lg_n+rate).  Output column-major [batches][2^(lg_n+rate)]. */
+void orc_lde(u64 *out, const u64 *in, unsigned lg_n, unsigned rate, unsigned batches,
+             int inverse, int coset, u64 gen)
+{
+    u64 n = 1ull << lg_n, N = n << rate;
+    for (u64 b = 0; b < batches; b++) {
+        memcpy(out + b * N, in + b * n, n * sizeof(u64));
+        memset(out + b * N + n, 0, (N - n) * sizeof(u64));
+    }
+    orc_ntt(out, lg_n + rate, batches, inverse, coset, gen);
+}
+
+/* compute_transpose_rev semantics (kernels.cu:94-135):
+ * out[t * batches + b] = in[b * 2^lg_n + bitrev_lg_n(t)]. */
+void orc_transpose_rev(u64 *out, const u64 *in, unsigned lg_n, unsigned batches)
+{
+    u64 n = 1ull << lg_n;
+#pragma omp parallel for schedule(static)
+    for (long long t = 0; t < (long long)n; t++) {
+        u64 src = bitrev((u64)t, lg_n);
+        for (u64 b = 0; b < batches; b++) out[t * batches + b] = in[b * n + src];
+    }
+}
+
+/* ---------------------------------------------------- synthetic inputs */
+
+/* splitmix64 stream rejection-sampled into [0, p): the canonical-element
+ * generator the tests, the bench and the fixtures share. */
+void orc_fill_canonical(u64 *out, u64 count, u64 seed)
+{
+    u64 x = seed;
+    for (u64 i = 0; i < count;) {
+        x += 0x9E3779B97F4A7C15ull;
+        u64 z = x;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        if (z < GL_P) out[i++] = z;
+    }
+}
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

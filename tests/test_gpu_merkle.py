@@ -1,0 +1,87 @@
+"""GPU Merkle forest through the C ABI vs the oracle and the reference KATs.
+Bit-exact (tolerance zero)."""
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from test_oracle import FIB12, GO_CAPS, GO_LEAF7, LEAF9, POSEIDON2_KAT, POSEIDON_KAT
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_tree(zk, leaves, ls, h, ht):
+    return zk.merkle_tree(leaves, ls, h, ht)
+
+
+def test_poseidon_kat_direct(zk):
+    # caps == leaves: cap[i] = hash_one(leaf i); one leaf per k = 1..9
+    for k in range(1, 10):
+        _, caps = gpu_tree(zk, np.array(LEAF9[:k], np.uint64), k, 0, 0)
+        assert [int(v) for v in caps] == POSEIDON_KAT[k]
+
+
+def test_poseidon2_kat_direct(zk):
+    _, caps = gpu_tree(zk, np.array(FIB12, np.uint64), 12, 0, 3)
+    assert [int(v) for v in caps] == POSEIDON2_KAT
+
+
+@pytest.mark.parametrize("nl,nc,cap", GO_CAPS)
+def test_go_cap_kats(zk, nl, nc, cap):
+    leaves = np.array(GO_LEAF7 * nl, dtype=np.uint64)
+    _, caps = gpu_tree(zk, leaves, 7, nc.bit_length() - 1, 0)
+    for j in range(nc):
+        assert [int(v) for v in caps[4 * j:4 * j + 4]] == cap
+
+
+SHAPES = [(1, 7, 0), (1, 7, 1), (2, 9, 1), (3, 5, 1), (4, 2, 2), (5, 4, 3), (6, 3, 2), (7, 8, 0), (8, 16, 4),
+          (8, 17, 8), (9, 1, 4), (10, 135, 4), (11, 20, 0), (12, 80, 4), (9, 12, 9), (10, 400, 2)]
+
+
+@pytest.mark.parametrize("ht", [0, 3])
+@pytest.mark.parametrize("lg_l,ls,h", SHAPES)
+def test_tree_vs_oracle(zk, ht, lg_l, ls, h):
+    print(f"tree lg_l={lg_l} ls={ls} h={h} ht={ht}", flush=True)
+    leaves = ob.canonical(ls << lg_l, 1000 * lg_l + 10 * ls + h + ht)
+    dig, caps = gpu_tree(zk, leaves, ls, h, ht)
+    edig, ecap = ob.merkle(leaves, ls, h, ht)
+    assert np.array_equal(caps, ecap)
+    assert np.array_equal(dig, edig)
+
+
+@pytest.mark.parametrize("ht", [0, 3])
+@pytest.mark.parametrize("lg_l,ls,h", [(13, 135, 4),   # 2^13 x 135 leaves, cap height 4
+                                       (17, 5, 3),     # two passes over HBM
+                                       (18, 5, 0)])    # three passes over HBM
+def test_large_tree_vs_oracle(zk, ht, lg_l, ls, h):
+    print(f"large tree lg_l={lg_l} ls={ls} h={h} ht={ht}", flush=True)
+    leaves = ob.canonical(ls << lg_l, 77 + lg_l + ht)
+    dig, caps = gpu_tree(zk, leaves, ls, h, ht)
+    edig, ecap = ob.merkle(leaves, ls, h, ht)
+    assert np.array_equal(caps, ecap)
+    assert np.array_equal(dig, edig)
+
+
+def test_cpu_entry_point_runs_on_gpu(zk):
+    ls, lg_l, h = 9, 8, 2
+    leaves = ob.canonical(ls << lg_l, 5)
+    nl, nc = 1 << lg_l, 1 << h
+    nd = 2 * (nl - nc)
+    dig = np.zeros(nd * 4, np.uint64)
+    cap = np.zeros(nc * 4, np.uint64)
+    zk.fill_digests_buf_linear_cpu(dig, cap, leaves, nd, nc, nl, ls, h, 0)
+    edig, ecap = ob.merkle(leaves, ls, h, 0)
+    assert np.array_equal(cap, ecap) and np.array_equal(dig, edig)
+
+
+def test_invalid_arguments_are_reported(zk):
+    d = zk.DeviceBuffer(64)
+    c = zk.DeviceBuffer(16)
+    l = zk.DeviceBuffer(256)
+    c.zero()
+    with pytest.raises(zk.ZeknoxError):  # digests size inconsistent
+        zk.fill_digests_buf_linear_gpu_with_gpu_ptr(d, c, l, 10, 4, 32, 8, 2, 0)
+    with pytest.raises(zk.ZeknoxError):  # leaves not a power of two
+        zk.fill_digests_buf_linear_gpu_with_gpu_ptr(d, c, l, 2 * (24 - 4), 4, 24, 8, 2, 0)
+    with pytest.raises(zk.ZeknoxError):  # hash type not provided
+        zk.fill_digests_buf_linear_gpu_with_gpu_ptr(d, c, l, 2 * (32 - 4), 4, 32, 8, 2, 1)
+    assert np.all(c.to_host() == 0)  # nothing written

@@ -1,0 +1,236 @@
+"""Pins the CPU oracle (oracle/oracle.c) before anything is compared to it.
+
+Every function of the oracle is checked against something that does not
+restate its own formula: the reference's known-answer tests, the reference's
+own CPU code built from its sources (oracle/_ref, when present), brute force
+in Python big integers, or closed forms of the mathematics.
+"""
+import random
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from oracle_bind import P, orc
+
+# ---------------------------------------------------------------- field
+
+
+EDGE = [0, 1, 2, P - 1, P - 2, 2**32 - 1, 2**32, 2**32 + 1, 2**63, 2**63 - 1, P - 2**32, 0xFFFFFFFF00000000,
+        12345678901234567, 7]
+
+
+def test_field_ops_brute_force():
+    rng = random.Random(1)
+    vals = EDGE + [rng.randrange(P) for _ in range(200)]
+    for a in vals:
+        for b in vals[:40]:
+            assert orc.orc_gl_add(a, b) == (a + b) % P
+            assert orc.orc_gl_sub(a, b) == (a - b) % P
+            assert orc.orc_gl_mul(a, b) == (a * b) % P
+    for a in vals[1:60]:
+        inv = orc.orc_gl_inv(a)
+        assert a * inv % P == 1
+
+
+def test_omega_closed_forms():
+    # gl64_params.hpp:15-49 (OMEGA) -- a few table entries quoted as data
+    table = {0: 1, 1: 0xFFFFFFFF00000000, 2: 0x0001000000000000, 3: 0xFFFFFFFEFF000001,
+             4: 0xEFFFFFFF00000001, 5: 0x00003FFFFFFFC000, 6: 0x0000008000000000,
+             32: 1753635133440165772}
+    for k, v in table.items():
+        assert orc.orc_gl_omega(k) == v
+    for k in range(1, 33):
+        w = orc.orc_gl_omega(k)
+        assert pow(w, 2**k, P) == 1
+        assert pow(w, 2 ** (k - 1), P) == P - 1  # exact order 2^k
+        assert w * w % P == orc.orc_gl_omega(k - 1)
+
+
+# ------------------------------------------------------------- hash KATs
+
+LEAF9 = [8395359103262935841, 1377884553022145855, 2370707998790318766, 3651132590097252162,
+         1141848076261006345, 12736915248278257710, 9898074228282442027, 16154511938222758243,
+         3651132590097252162]
+
+# native/tests/tests.cu:281-292: PoseidonHasher::cpu_hash_one(leaf, k) for k = 1..9
+POSEIDON_KAT = {
+    1: [8395359103262935841, 0, 0, 0],
+    2: [8395359103262935841, 1377884553022145855, 0, 0],
+    3: [8395359103262935841, 1377884553022145855, 2370707998790318766, 0],
+    4: [8395359103262935841, 1377884553022145855, 2370707998790318766, 3651132590097252162],
+    5: [3618821072812614426, 8353148445756493727, 4040525329700581442, 15983474240847269257],
+    6: [16643938361881363776, 6653675298471110559, 12562058402463703932, 16154511938222758243],
+    7: [7544909477878586743, 7431000548126831493, 17815668806142634286, 13168106265494210017],
+    8: [6835933650993053111, 15978194778874965616, 2024081381896137659, 16520693669262110264],
+    9: [9429914239539731992, 14881719063945231827, 15528667124986963891, 16465743531992249573],
+}
+
+# native/tests/tests.cu:425-428: Poseidon2 of the Fibonacci 12-vector
+FIB12 = [0, 1, 1, 2, 3, 5, 8, 13, 21, 34, 55, 89]
+POSEIDON2_KAT = [0x133A03ECA11D93FB, 0x5365414FB618F58D, 0xFA49F50F3A2BA2E5, 0xD16E53672C9832A4]
+
+# wrappers/go/lib/lib_test.go:197-213: caps of trees of identical 7-element leaves
+GO_LEAF7 = LEAF9[:7]
+GO_CAPS = [
+    (8, 8, [7544909477878586743, 7431000548126831493, 17815668806142634286, 13168106265494210017]),
+    (8, 4, [7320113784629306061, 6779895905614599055, 11363085033710007978, 10376477785629036773]),
+    (16, 2, [15643713602922591410, 15764162833178839023, 7963799186968795376, 10933634831850922421]),
+]
+
+
+@pytest.mark.parametrize("k", range(1, 10))
+def test_poseidon_kat(k):
+    assert [int(v) for v in ob.hash_one(LEAF9[:k], 0)] == POSEIDON_KAT[k]
+
+
+def test_poseidon2_kat():
+    assert [int(v) for v in ob.hash_one(FIB12, 3)] == POSEIDON2_KAT
+
+
+@pytest.mark.parametrize("nl,nc,cap", GO_CAPS)
+def test_go_merkle_cap_kats(nl, nc, cap):
+    leaves = np.array(GO_LEAF7 * nl, dtype=np.uint64)
+    _, caps = ob.merkle(leaves, 7, nc.bit_length() - 1, 0)
+    for j in range(nc):
+        assert [int(v) for v in caps[4 * j:4 * j + 4]] == cap
+
+
+def test_merkle_layout_invariant():
+    """Heap layout: digest i of a subtree = H(d[2i+2], d[2i+3]); leaf hashes at
+    [sl-2, 2sl-2); cap = H(d[0], d[1])."""
+    ls, lg_l, h = 9, 6, 2
+    leaves = ob.canonical(ls << lg_l, 99)
+    dig, cap = ob.merkle(leaves, ls, h, 0)
+    nl, nc = 1 << lg_l, 1 << h
+    sl, sdl = nl // nc, 2 * (nl // nc) - 2
+    d = dig.reshape(-1, 4)
+    for s in range(nc):
+        base = s * sdl
+        for i in range(sl):
+            assert np.array_equal(d[base + sl - 2 + i], ob.hash_one(leaves[(s * sl + i) * ls:(s * sl + i + 1) * ls]))
+        for i in range(sl - 2):
+            assert np.array_equal(d[base + i], ob.hash_two(d[base + 2 * i + 2], d[base + 2 * i + 3]))
+        assert np.array_equal(cap[4 * s:4 * s + 4], ob.hash_two(d[base], d[base + 1]))
+
+
+# --------------------------------------------- against the reference's code
+
+ref = ob.ref_lib()
+needs_ref = pytest.mark.skipif(ref is None, reason="oracle/_ref not built (reference sources absent)")
+
+
+@needs_ref
+@pytest.mark.parametrize("ht", [0, 3])
+def test_hash_vs_reference(ht):
+    rng = np.random.default_rng(5)
+    for n in list(range(0, 26)) + [135, 400]:
+        x = ob.canonical(max(n, 1), 1000 + n)[:n] if n else np.zeros(0, np.uint64)
+        exp = np.zeros(4, np.uint64)
+        xx = x if n else np.zeros(1, np.uint64)
+        ref.ref_hash_one(ob._ptr(xx), n, ob._ptr(exp), ht)
+        assert np.array_equal(ob.hash_one(x, ht), exp), n
+    for _ in range(20):
+        l, r = ob.canonical(4, int(rng.integers(1 << 40))), ob.canonical(4, int(rng.integers(1 << 40)))
+        exp = np.zeros(4, np.uint64)
+        ref.ref_hash_two(ob._ptr(l), ob._ptr(r), ob._ptr(exp), ht)
+        assert np.array_equal(ob.hash_two(l, r, ht), exp)
+
+
+@needs_ref
+@pytest.mark.parametrize("ht", [0, 3])
+@pytest.mark.parametrize("lg_l,ls,h", [(1, 5, 0), (3, 7, 1), (4, 2, 2), (5, 9, 5), (6, 8, 3), (7, 20, 4),
+                                       (8, 135, 4), (6, 4, 1), (5, 3, 2), (6, 1, 0)])
+def test_merkle_vs_reference(ht, lg_l, ls, h):
+    leaves = ob.canonical(ls << lg_l, 7 + lg_l * 31 + ls)
+    nl, nc = 1 << lg_l, 1 << h
+    nd = 2 * (nl - nc) if nl != nc else 0
+    rd = np.zeros(max(nd, 1) * 4, np.uint64)
+    rc = np.zeros(nc * 4, np.uint64)
+    ref.fill_digests_buf_linear_cpu(ob._ptr(rd), ob._ptr(rc), ob._ptr(leaves), nd, nc, nl, ls, h, ht)
+    dig, cap = ob.merkle(leaves, ls, h, ht)
+    assert np.array_equal(cap, rc)
+    assert np.array_equal(dig, rd[:nd * 4])
+
+
+# ------------------------------------------------------------- NTT / LDE
+
+
+def dft(coeffs, lg, inverse=False, coset=False, gen=7):
+    """O(n^2) definition in Python integers."""
+    n = 1 << lg
+    w = orc.orc_gl_omega(lg)
+    if inverse:
+        w = pow(w, P - 2, P)
+    a = [int(v) for v in coeffs]
+    if coset and not inverse:
+        a = [a[i] * pow(gen, i, P) % P for i in range(n)]
+    out = [sum(a[i] * pow(w, i * j, P) for i in range(n)) % P for j in range(n)]
+    if inverse:
+        ninv = pow(n, P - 2, P)
+        out = [(v * ninv % P) * (pow(gen, j, P) if coset else 1) % P for j, v in enumerate(out)]
+    return out
+
+
+@pytest.mark.parametrize("lg", [1, 2, 3, 5, 6])
+@pytest.mark.parametrize("inverse", [False, True])
+@pytest.mark.parametrize("coset", [False, True])
+def test_ntt_brute_force(lg, inverse, coset):
+    x = ob.canonical(2 << lg, 3 * lg + inverse + 5 * coset)
+    got = ob.ntt(x, lg, 2, inverse, coset)
+    for b in range(2):
+        assert [int(v) for v in got[b << lg:(b + 1) << lg]] == dft(x[b << lg:(b + 1) << lg], lg, inverse, coset)
+
+
+def test_ntt_roundtrip_and_linearity():
+    lg, B = 10, 3
+    x = ob.canonical(B << lg, 11)
+    y = ob.canonical(B << lg, 12)
+    fx = ob.ntt(x, lg, B)
+    assert np.array_equal(ob.ntt(fx, lg, B, inverse=True), x)
+    s = np.array([(int(a) + int(b)) % P for a, b in zip(x, y)], dtype=np.uint64)
+    fs = ob.ntt(s, lg, B)
+    fy = ob.ntt(y, lg, B)
+    assert all(int(fs[i]) == (int(fx[i]) + int(fy[i])) % P for i in range(0, B << lg, 37))
+
+
+@pytest.mark.parametrize("lg_n,rate", [(1, 1), (3, 2), (4, 3), (6, 1)])
+def test_lde_brute_force(lg_n, rate):
+    x = ob.canonical(2 << lg_n, lg_n * 10 + rate)
+    N = 1 << (lg_n + rate)
+    got = ob.lde(x, lg_n, rate, 2)
+    for b in range(2):
+        padded = [int(v) for v in x[b << lg_n:(b + 1) << lg_n]] + [0] * (N - (1 << lg_n))
+        assert [int(v) for v in got[b * N:(b + 1) * N]] == dft(padded, lg_n + rate, coset=True)
+
+
+def test_lde_closed_forms():
+    lg_n, rate = 8, 2
+    N = 1 << (lg_n + rate)
+    one = np.zeros(1 << lg_n, np.uint64)
+    one[0] = 1
+    assert np.all(ob.lde(one, lg_n, rate, 1) == 1)  # p(x) = 1
+    xpoly = np.zeros(1 << lg_n, np.uint64)
+    xpoly[1] = 1  # p(x) = x  ->  g * w^j
+    out = ob.lde(xpoly, lg_n, rate, 1)
+    w = orc.orc_gl_omega(lg_n + rate)
+    assert all(int(out[j]) == 7 * pow(w, j, P) % P for j in range(0, N, 13))
+    # Horner spot checks of a random polynomial
+    c = [int(v) for v in ob.canonical(1 << lg_n, 77)]
+    out = ob.lde(np.array(c, dtype=np.uint64), lg_n, rate, 1)
+    for j in [0, 1, 5, N // 2, N - 1]:
+        pt = 7 * pow(w, j, P) % P
+        acc = 0
+        for a in reversed(c):
+            acc = (acc * pt + a) % P
+        assert int(out[j]) == acc
+
+
+def test_transpose_rev_brute_force():
+    lg, B = 5, 7
+    x = ob.canonical(B << lg, 4)
+    out = ob.transpose_rev(x, lg, B)
+    n = 1 << lg
+    for t in range(n):
+        for b in range(B):
+            assert out[t * B + b] == x[b * n + ob.bitrev(t, lg)]

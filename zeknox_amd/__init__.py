@@ -1,0 +1,281 @@
+"""Python binding of libzeknox (the MI355X LDE + Merkle-commit library).
+
+Mirrors the reference's Rust wrapper (wrappers/rust/src/lib.rs:99-260): the
+same function names and argument meaning, with failures raised as
+ZeknoxError where the Rust wrapper panics.  Buffers are numpy uint64 arrays
+(host) or DeviceBuffer objects (device memory owned by the library's HIP
+runtime, like the wrapper's cudart-backed device slices).
+
+The library must have been built (zeknox_amd.build.build()); importing this
+module loads it and fails loudly if it is missing -- there is no CPU
+fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libzeknox.so")
+
+P = 0xFFFFFFFF00000001
+HASH_POSEIDON = 0
+HASH_POSEIDON2 = 3
+FORWARD, INVERSE = 0, 1
+
+
+class ZeknoxError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"zeknox error {code}: {message}")
+        self.code = code
+        self.message = message
+
+
+class RustError(ctypes.Structure):
+    """native/utils/rusterror.h:15-41 -- {int code; char *message}."""
+    _fields_ = [("code", ctypes.c_int), ("message", ctypes.c_void_p)]
+
+
+class NTTConfig(ctypes.Structure):
+    """native/ntt/ntt.h:43-55 (24 bytes, passed by value)."""
+    _fields_ = [
+        ("batches", ctypes.c_uint32),
+        ("order", ctypes.c_int),
+        ("ntt_type", ctypes.c_int),
+        ("extension_rate_bits", ctypes.c_uint32),
+        ("are_inputs_on_device", ctypes.c_bool),
+        ("are_outputs_on_device", ctypes.c_bool),
+        ("with_coset", ctypes.c_bool),
+        ("is_multi_gpu", ctypes.c_bool),
+        ("salt_size", ctypes.c_uint32),
+    ]
+
+    def __init__(self, batches=1, extension_rate_bits=0, are_inputs_on_device=False,
+                 are_outputs_on_device=False, with_coset=False, salt_size=0):
+        super().__init__(batches, 0, 0, extension_rate_bits, are_inputs_on_device,
+                         are_outputs_on_device, with_coset, False, salt_size)
+
+
+class TransposeConfig(ctypes.Structure):
+    """native/ntt/ntt.h:57-62."""
+    _fields_ = [
+        ("batches", ctypes.c_uint32),
+        ("are_inputs_on_device", ctypes.c_bool),
+        ("are_outputs_on_device", ctypes.c_bool),
+    ]
+
+    def __init__(self, batches=1, are_inputs_on_device=False, are_outputs_on_device=False):
+        super().__init__(batches, are_inputs_on_device, are_outputs_on_device)
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run zeknox_amd.build.build() (hipcc, gfx950)")
+    lib = ctypes.CDLL(LIB_PATH)
+    u64, sz, u32, vp = ctypes.c_uint64, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_void_p
+    sig = {
+        "get_number_of_gpus": (RustError, [ctypes.POINTER(sz)]),
+        "list_devices_info": (RustError, []),
+        "init_cuda": (None, []),
+        "init_cuda_degree": (None, [u32]),
+        "init_cuda_degree_and_generator": (None, [u32, u64]),
+        "init_twiddle_factors": (RustError, [sz, sz]),
+        "init_coset": (RustError, [sz, sz, u64]),
+        "compute_batched_ntt": (RustError, [sz, vp, u32, ctypes.c_int, NTTConfig]),
+        "compute_batched_lde": (RustError, [sz, vp, vp, u32, ctypes.c_int, NTTConfig]),
+        "compute_batched_lde_multi_gpu": (RustError, [vp, vp, u32, ctypes.c_int, NTTConfig, u32, sz, sz]),
+        "compute_transpose_rev": (RustError, [sz, vp, vp, u32, TransposeConfig]),
+        "fill_digests_buf_linear_gpu_with_gpu_ptr": (None, [vp, vp, vp, u64, u64, u64, u64, u64, u64, u64]),
+        "fill_digests_buf_linear_multigpu_with_gpu_ptr": (None, [vp, vp, vp, u64, u64, u64, u64, u64, u64]),
+        "fill_digests_buf_linear_cpu": (None, [vp, vp, vp, u64, u64, u64, u64, u64, u64]),
+        "cuda_available": (ctypes.c_bool, []),
+        "zkx_last_error": (ctypes.c_char_p, []),
+        "zkx_device_malloc": (RustError, [sz, ctypes.POINTER(vp), sz]),
+        "zkx_device_free": (RustError, [sz, vp]),
+        "zkx_memcpy_htod": (RustError, [sz, vp, vp, sz]),
+        "zkx_memcpy_dtoh": (RustError, [sz, vp, vp, sz]),
+        "zkx_memset": (RustError, [sz, vp, ctypes.c_int, sz]),
+        "zkx_device_synchronize": (RustError, [sz]),
+        "zkx_stream": (vp, [sz]),
+        "zkx_profile_enable": (RustError, [ctypes.c_int]),
+        "zkx_kernel_stats": (ctypes.c_int, [vp, vp, vp, ctypes.c_int]),
+        "zkx_kernel_stats_reset": (None, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+_libc = ctypes.CDLL(None)
+_libc.free.argtypes = [ctypes.c_void_p]
+
+
+def _check(err):
+    if err.code != 0:
+        msg = ctypes.string_at(err.message).decode() if err.message else f"sppark::Error #{err.code}"
+        if err.message:
+            _libc.free(err.message)
+        raise ZeknoxError(err.code, msg)
+
+
+def last_error():
+    return lib.zkx_last_error().decode()
+
+
+class DeviceBuffer:
+    """Device allocation of `count` u64 on `device_id`."""
+
+    def __init__(self, count, device_id=0):
+        self.count = int(count)
+        self.device_id = device_id
+        p = ctypes.c_void_p()
+        _check(lib.zkx_device_malloc(device_id, ctypes.byref(p), self.count * 8))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, arr, device_id=0):
+        a = np.ascontiguousarray(arr, dtype=np.uint64).ravel()
+        buf = cls(max(a.size, 1), device_id)
+        buf.copy_from_host(a)
+        return buf
+
+    def copy_from_host(self, arr):
+        a = np.ascontiguousarray(arr, dtype=np.uint64).ravel()
+        assert a.size <= self.count
+        _check(lib.zkx_memcpy_htod(self.device_id, self.ptr, a.ctypes.data, a.size * 8))
+
+    def to_host(self, count=None):
+        n = self.count if count is None else int(count)
+        out = np.empty(n, dtype=np.uint64)
+        _check(lib.zkx_memcpy_dtoh(self.device_id, out.ctypes.data, self.ptr, n * 8))
+        return out
+
+    def zero(self):
+        _check(lib.zkx_memset(self.device_id, self.ptr, 0, self.count * 8))
+
+    def free(self):
+        if self.ptr:
+            _check(lib.zkx_device_free(self.device_id, self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _addr(x):
+    if isinstance(x, DeviceBuffer):
+        return x.ptr
+    if isinstance(x, np.ndarray):
+        assert x.dtype == np.uint64 and x.flags["C_CONTIGUOUS"]
+        return x.ctypes.data
+    return x  # raw address
+
+
+# ---- wrappers/rust/src/lib.rs:99-260 mirror -------------------------------
+
+def get_number_of_gpus_rs():
+    n = ctypes.c_size_t()
+    _check(lib.get_number_of_gpus(ctypes.byref(n)))
+    return n.value
+
+
+def list_devices_info_rs():
+    _check(lib.list_devices_info())
+
+
+def init_cuda_rs():
+    lib.init_cuda()
+
+
+def init_cuda_degree_rs(max_degree):
+    lib.init_cuda_degree(max_degree)
+
+
+def init_twiddle_factors_rs(device_id, lg_n):
+    _check(lib.init_twiddle_factors(device_id, lg_n))
+
+
+def init_coset_rs(device_id, lg_n, coset_gen):
+    _check(lib.init_coset(device_id, lg_n, coset_gen))
+
+
+def lde_batch(device_id, output, input, log_n_size, cfg, direction=FORWARD):
+    _check(lib.compute_batched_lde(device_id, _addr(output), _addr(input), log_n_size, direction, cfg))
+
+
+def lde_batch_multi_gpu(output, input, num_gpu, cfg, log_n_size, total_in, total_out):
+    _check(lib.compute_batched_lde_multi_gpu(_addr(output), _addr(input), num_gpu, FORWARD, cfg, log_n_size,
+                                             total_in, total_out))
+
+
+def ntt_batch(device_id, inout, log_n_size, cfg):
+    _check(lib.compute_batched_ntt(device_id, _addr(inout), log_n_size, FORWARD, cfg))
+
+
+def intt_batch(device_id, inout, log_n_size, cfg):
+    _check(lib.compute_batched_ntt(device_id, _addr(inout), log_n_size, INVERSE, cfg))
+
+
+def transpose_rev_batch(device_id, output, input, log_n_size, cfg):
+    _check(lib.compute_transpose_rev(device_id, _addr(output), _addr(input), log_n_size, cfg))
+
+
+def fill_digests_buf_linear_gpu_with_gpu_ptr(digests, caps, leaves, digests_size, caps_size, leaves_size,
+                                             leaf_size, cap_height, hash_type, gpu_id=0):
+    """merkle.h:30-40; raises ZeknoxError on the argument errors the
+    reference asserts on."""
+    lib.fill_digests_buf_linear_gpu_with_gpu_ptr(_addr(digests), _addr(caps), _addr(leaves), digests_size,
+                                                 caps_size, leaves_size, leaf_size, cap_height, hash_type, gpu_id)
+    e = last_error()
+    if e:
+        raise ZeknoxError(1, e)
+
+
+def fill_digests_buf_linear_cpu(digests, caps, leaves, digests_size, caps_size, leaves_size, leaf_size,
+                                cap_height, hash_type):
+    lib.fill_digests_buf_linear_cpu(_addr(digests), _addr(caps), _addr(leaves), digests_size, caps_size,
+                                    leaves_size, leaf_size, cap_height, hash_type)
+    e = last_error()
+    if e:
+        raise ZeknoxError(1, e)
+
+
+def merkle_tree(leaves, leaf_size, cap_height, hash_type=HASH_POSEIDON, device_id=0):
+    """Convenience: host leaves (n_leaves x leaf_size) -> (digests, caps) via
+    the device entry point."""
+    leaves = np.ascontiguousarray(leaves, dtype=np.uint64).ravel()
+    nl = leaves.size // leaf_size
+    nc = 1 << cap_height
+    nd = 2 * (nl - nc) if nc != nl else 0
+    dl = DeviceBuffer.from_host(leaves, device_id)
+    dc = DeviceBuffer(nc * 4, device_id)
+    dd = DeviceBuffer(max(nd, 1) * 4, device_id)
+    fill_digests_buf_linear_gpu_with_gpu_ptr(dd, dc, dl, nd, nc, nl, leaf_size, cap_height, hash_type, device_id)
+    return (dd.to_host(nd * 4) if nd else np.zeros(0, np.uint64)), dc.to_host()
+
+
+# ---- timing ---------------------------------------------------------------
+
+def profile_enable(on=True):
+    _check(lib.zkx_profile_enable(1 if on else 0))
+
+
+def kernel_stats():
+    """{family: (launches, total_ms)} of library kernels timed with HIP
+    events on their launch stream since the last reset."""
+    cap = 64
+    names = (ctypes.c_char * 32 * cap)()
+    launches = (ctypes.c_uint64 * cap)()
+    totals = (ctypes.c_double * cap)()
+    n = lib.zkx_kernel_stats(names, launches, totals, cap)
+    return {names[i].value.decode(): (launches[i], totals[i]) for i in range(min(n, cap))}
+
+
+def kernel_stats_reset():
+    lib.zkx_kernel_stats_reset()

@@ -1,0 +1,275 @@
+// The exported C ABI (include/zeknox.h): argument checks, host/device
+// buffer handling and error mapping around the drivers in ntt.hip and
+// merkle.hip.  Mirrors native/lib.cu:36-175 and the entry points of
+// native/ntt/ntt.cu and native/merkle/merkle.cu.
+#include <stdio.h>
+#include <string.h>
+#include <time.h>
+
+#include <string>
+
+#include "../../include/zeknox.h"
+#include "merkle.h"
+#include "ntt.h"
+
+using namespace zkx;
+
+namespace {
+
+RustError ok() { return RustError{0, nullptr}; }
+
+RustError err(int code, const char *msg) { return RustError{code, msg ? strdup(msg) : nullptr}; }
+
+template <class F>
+RustError guarded(F &&f)
+{
+    try {
+        f();
+        return ok();
+    } catch (const Error &e) {
+        return err(e.code, e.what());
+    } catch (const std::exception &e) {
+        return err(-1, e.what());
+    }
+}
+
+// Device view of a caller buffer: the pointer itself when it is on the
+// device, else a library allocation (copied in when `copy_in`).
+struct View {
+    DevBuf own;
+    uint64_t *p = nullptr;
+    View(void *user, bool on_device, size_t elems, bool copy_in, Device &d)
+    {
+        if (on_device) {
+            p = static_cast<uint64_t *>(user);
+        } else {
+            own = DevBuf(elems * sizeof(uint64_t));
+            p = own.as<uint64_t>();
+            if (copy_in && elems)
+                ZKX_HIP(hipMemcpyAsync(p, user, elems * sizeof(uint64_t), hipMemcpyHostToDevice, d.stream));
+        }
+    }
+};
+
+void copy_out(void *user, bool on_device, const uint64_t *dev, size_t elems, Device &d)
+{
+    if (!on_device && elems)
+        ZKX_HIP(hipMemcpyAsync(user, dev, elems * sizeof(uint64_t), hipMemcpyDeviceToHost, d.stream));
+}
+
+void lde_entry(size_t device_id, void *output, void *input, uint32_t lg_n, NTT_Direction dir, NTT_Config cfg)
+{
+    if (lg_n == 0 || cfg.extension_rate_bits < 1) invalid("invalid args");
+    if (!output || !input) invalid("null buffer");
+    const uint32_t lg = lg_n + cfg.extension_rate_bits;
+    if (lg > 32) invalid("extended log size exceeds 32");
+    Device &d = device(device_id);
+    std::lock_guard<std::mutex> g(d.mu);
+    const size_t n_in = (size_t)cfg.batches << lg_n;
+    const size_t n_out = ((size_t)cfg.batches + cfg.salt_size) << lg;
+    View in(input, cfg.are_inputs_on_device, n_in, true, d);
+    View out(output, cfg.are_outputs_on_device, n_out, false, d);
+    if (cfg.batches) lde(d, out.p, in.p, lg_n, cfg.extension_rate_bits, cfg.batches, dir == inverse, cfg.with_coset);
+    if (cfg.salt_size) {
+        const uint64_t count = (uint64_t)cfg.salt_size << lg;
+        const uint64_t seed = (uint64_t)time(nullptr) * 0x9E3779B97F4A7C15ull ^ (uint64_t)clock();
+        fill_salt(out.p + ((size_t)cfg.batches << lg), count, seed, d.stream);
+    }
+    copy_out(output, cfg.are_outputs_on_device, out.p, n_out, d);
+    d.sync();
+}
+
+void merkle_entry(void *dig, void *cap, const void *leaves, uint64_t nd, uint64_t nc, uint64_t nl, uint64_t ls,
+                  uint64_t h, uint64_t ht, uint64_t gpu_id, bool host_buffers)
+{
+    MerkleSpec s;
+    s.num_digests = nd;
+    s.num_caps = nc;
+    s.num_leaves = nl;
+    s.leaf_size = ls;
+    s.cap_height = h;
+    s.hash_type = ht;
+    s.leaf_stride = ls;
+    s.col_stride = 1;
+    s.digests = static_cast<uint64_t *>(dig);
+    s.caps = static_cast<uint64_t *>(cap);
+    s.leaves = static_cast<const uint64_t *>(leaves);
+    check_merkle(s);
+    Device &d = device(gpu_id);
+    std::lock_guard<std::mutex> g(d.mu);
+    if (!host_buffers) {
+        merkle_build(d, s);
+        d.sync();
+        return;
+    }
+    const bool direct = nc == nl;
+    DevBuf dl(nl * ls * sizeof(uint64_t)), dc(nc * 4 * sizeof(uint64_t)), dd(direct ? 0 : nd * 4 * sizeof(uint64_t));
+    ZKX_HIP(hipMemcpyAsync(dl.p, leaves, dl.bytes, hipMemcpyHostToDevice, d.stream));
+    s.leaves = dl.as<uint64_t>();
+    s.caps = dc.as<uint64_t>();
+    s.digests = direct ? nullptr : dd.as<uint64_t>();
+    merkle_build(d, s);
+    ZKX_HIP(hipMemcpyAsync(cap, dc.p, dc.bytes, hipMemcpyDeviceToHost, d.stream));
+    if (!direct && dd.bytes) ZKX_HIP(hipMemcpyAsync(dig, dd.p, dd.bytes, hipMemcpyDeviceToHost, d.stream));
+    d.sync();
+}
+
+void merkle_void(void *dig, void *cap, const void *leaves, uint64_t nd, uint64_t nc, uint64_t nl, uint64_t ls,
+                 uint64_t h, uint64_t ht, uint64_t gpu_id, bool host_buffers)
+{
+    try {
+        merkle_entry(dig, cap, leaves, nd, nc, nl, ls, h, ht, gpu_id, host_buffers);
+        set_last_error("");
+    } catch (const std::exception &e) {
+        set_last_error(e.what());
+        fprintf(stderr, "zeknox: fill_digests_buf_linear: %s\n", e.what());
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+RustError get_number_of_gpus(size_t *ngpus)
+{
+    return guarded([&] {
+        if (!ngpus) invalid("null pointer");
+        *ngpus = (size_t)device_count();
+    });
+}
+
+RustError list_devices_info(void)
+{
+    return guarded([&] {
+        for (int i = 0; i < device_count(); i++) {
+            Device &d = device((size_t)i);
+            const hipDeviceProp_t &p = d.prop;
+            printf("Device %d: %s (%s)\n", i, p.name, p.gcnArchName);
+            printf("  compute units        : %d\n", p.multiProcessorCount);
+            printf("  clock (kHz)          : %d\n", p.clockRate);
+            printf("  global memory (MiB)  : %zu\n", (size_t)(p.totalGlobalMem >> 20));
+            printf("  LDS per block (KiB)  : %zu\n", (size_t)(p.sharedMemPerBlock >> 10));
+            printf("  L2 cache (KiB)       : %d\n", p.l2CacheSize >> 10);
+            printf("  wavefront size       : %d\n", p.warpSize);
+        }
+    });
+}
+
+bool cuda_available(void) { return device_count() > 0; }
+
+RustError init_twiddle_factors(size_t device_id, size_t lg_n)
+{
+    return guarded([&] {
+        if (lg_n > 32) invalid("log size exceeds 32");
+        Device &d = device(device_id);
+        std::lock_guard<std::mutex> g(d.mu);
+        d.twiddle_table((uint32_t)lg_n);
+        d.sync();
+    });
+}
+
+RustError init_coset(size_t device_id, size_t lg_domain_size, const uint64_t coset_gen)
+{
+    return guarded([&] {
+        if (lg_domain_size > 32) invalid("log size exceeds 32");
+        Device &d = device(device_id);
+        std::lock_guard<std::mutex> g(d.mu);
+        d.build_coset((uint32_t)lg_domain_size, coset_gen % 0xFFFFFFFF00000001ull);
+        d.sync();
+    });
+}
+
+void init_cuda_degree_and_generator(const uint32_t max_degree, const uint64_t group_generator)
+{
+    for (int i = 0; i < device_count(); i++) {
+        init_coset((size_t)i, max_degree, group_generator);
+        for (uint32_t k = 2; k <= max_degree; k++) init_twiddle_factors((size_t)i, k);
+    }
+}
+
+void init_cuda_degree(const uint32_t max_degree) { init_cuda_degree_and_generator(max_degree, 7); }
+
+void init_cuda(void) { init_cuda_degree(24); }
+
+RustError compute_batched_ntt(size_t device_id, void *inout, uint32_t lg_domain_size, NTT_Direction ntt_direction,
+                              NTT_Config cfg)
+{
+    return guarded([&] {
+        if (lg_domain_size == 0 || lg_domain_size > 32) invalid("invalid args");
+        if (!inout) invalid("null buffer");
+        Device &d = device(device_id);
+        std::lock_guard<std::mutex> g(d.mu);
+        const size_t elems = (size_t)cfg.batches << lg_domain_size;
+        View v(inout, cfg.are_inputs_on_device, elems, true, d);
+        if (cfg.batches) ntt(d, v.p, lg_domain_size, cfg.batches, ntt_direction == inverse, cfg.with_coset);
+        // results go back to the caller's buffer unless it asked to keep
+        // them on the device (ntt.cu:290-297)
+        if (!cfg.are_inputs_on_device) copy_out(inout, false, v.p, elems, d);
+        d.sync();
+    });
+}
+
+RustError compute_batched_lde(size_t device_id, void *output, void *input, uint32_t lg_domain_size,
+                              NTT_Direction ntt_direction, NTT_Config cfg)
+{
+    return guarded([&] { lde_entry(device_id, output, input, lg_domain_size, ntt_direction, cfg); });
+}
+
+RustError compute_batched_lde_multi_gpu(void *output, void *input, uint32_t num_gpu, NTT_Direction ntt_direction,
+                                        NTT_Config cfg, uint32_t lg_domain_size, size_t total_num_input_elements,
+                                        size_t total_num_output_elements)
+{
+    return guarded([&] {
+        if (total_num_input_elements == 0 || lg_domain_size == 0 || cfg.extension_rate_bits < 1)
+            invalid("invalid args");
+        if (num_gpu != 1) invalid("this build drives one device: num_gpu must be 1");
+        if (total_num_input_elements != ((size_t)cfg.batches << lg_domain_size) ||
+            total_num_output_elements < (((size_t)cfg.batches + cfg.salt_size) << (lg_domain_size + cfg.extension_rate_bits)))
+            invalid("element counts do not match the configuration");
+        lde_entry(0, output, input, lg_domain_size, ntt_direction, cfg);
+    });
+}
+
+RustError compute_transpose_rev(size_t device_id, void *output, void *input, uint32_t lg_n, NTT_TransposeConfig cfg)
+{
+    return guarded([&] {
+        if (lg_n > 32) invalid("log size exceeds 32");
+        if (!output || !input) invalid("null buffer");
+        Device &d = device(device_id);
+        std::lock_guard<std::mutex> g(d.mu);
+        const size_t elems = (size_t)cfg.batches << lg_n;
+        View in(input, cfg.are_inputs_on_device, elems, true, d);
+        View out(output, cfg.are_outputs_on_device, elems, false, d);
+        if (elems) transpose_rev(d, out.p, in.p, lg_n, cfg.batches, true);
+        copy_out(output, cfg.are_outputs_on_device, out.p, elems, d);
+        d.sync();
+    });
+}
+
+void fill_digests_buf_linear_gpu_with_gpu_ptr(void *digests_buf_gpu_ptr, void *cap_buf_gpu_ptr,
+                                              void *leaves_buf_gpu_ptr, uint64_t digests_buf_size,
+                                              uint64_t cap_buf_size, uint64_t leaves_buf_size, uint64_t leaf_size,
+                                              uint64_t cap_height, uint64_t hash_type, uint64_t gpu_id)
+{
+    merkle_void(digests_buf_gpu_ptr, cap_buf_gpu_ptr, leaves_buf_gpu_ptr, digests_buf_size, cap_buf_size,
+                leaves_buf_size, leaf_size, cap_height, hash_type, gpu_id, false);
+}
+
+void fill_digests_buf_linear_multigpu_with_gpu_ptr(void *digests_buf_gpu_ptr, void *cap_buf_gpu_ptr,
+                                                   void *leaves_buf_gpu_ptr, uint64_t digests_buf_size,
+                                                   uint64_t cap_buf_size, uint64_t leaves_buf_size,
+                                                   uint64_t leaf_size, uint64_t cap_height, uint64_t hash_type)
+{
+    merkle_void(digests_buf_gpu_ptr, cap_buf_gpu_ptr, leaves_buf_gpu_ptr, digests_buf_size, cap_buf_size,
+                leaves_buf_size, leaf_size, cap_height, hash_type, 0, false);
+}
+
+void fill_digests_buf_linear_cpu(void *digests_buf_ptr, void *cap_buf_ptr, const void *leaves_buf_ptr,
+                                 uint64_t digests_buf_size, uint64_t cap_buf_size, uint64_t leaves_buf_size,
+                                 uint64_t leaf_size, uint64_t cap_height, uint64_t hash_type)
+{
+    merkle_void(digests_buf_ptr, cap_buf_ptr, leaves_buf_ptr, digests_buf_size, cap_buf_size, leaves_buf_size,
+                leaf_size, cap_height, hash_type, 0, true);
+}
+
+}  // extern "C"

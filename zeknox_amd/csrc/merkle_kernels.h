@@ -1,0 +1,125 @@
+// Linear Merkle forest kernels (merkle/merkle.cu:165-244 layout, built
+// MI355X-first).
+//
+// Layout (merkle/merkle.h, native/README.md:64-71): L leaves, 2^h caps,
+// 2(L - 2^h) digests of 4 u64.  Subtree s (sl = L >> h leaves) owns digests
+// [s*sdl, (s+1)*sdl), sdl = 2*sl - 2; inside it digest i is heap node i+2,
+// so node i has children 2i+2 and 2i+3, the level of width w sits at
+// [w-2, 2w-2), leaf hashes at [sl-2, 2sl-2), cap[s] = H(d_s[0], d_s[1]).
+//
+// Schedule.  A work-group of up to 256 lanes owns a tile of consecutive
+// nodes of one level (leaves in the first pass).  Each lane hashes its leaf
+// (or loads its node), then the tile is reduced level by level in LDS: every
+// level's digests are written to HBM on the way up, and a tile that reaches
+// the width-2 top level of its subtree also writes the cap.  A tile that
+// stops below the top leaves its root for the next pass, which starts from
+// that level; passes repeat until the caps are written (one pass when a
+// subtree has <= 256 leaves, two up to 2^16, three up to 2^24).
+#pragma once
+#include "hash.h"
+
+namespace zkx {
+
+constexpr int kMerkleThreads = 256;
+
+struct MerkleArgs {
+    const uint64_t *leaves;  // pass 0: leaf source
+    uint64_t *dig;
+    uint64_t *cap;
+    uint64_t count;          // nodes (or leaves) at the input level, all subtrees
+    uint64_t sdl;            // digests per subtree
+    uint32_t lg_w;           // log2 width of the input level per subtree
+    uint32_t tile_log;       // log2 nodes per work-group tile
+    uint32_t leaf_size;      // pass 0
+    uint64_t leaf_stride;    // u64 between consecutive leaves (row-major: leaf_size)
+    uint64_t col_stride;     // u64 between consecutive elements of a leaf (row-major: 1)
+};
+
+// Digest slot of node `idx` of the level of width 2^lg_w in subtree `sub`.
+__device__ __forceinline__ uint64_t *node_ptr(const MerkleArgs &a, uint64_t sub, uint32_t lg_w, uint64_t idx)
+{
+    return a.dig + 4 * (sub * a.sdl + ((1ull << lg_w) - 2) + idx);
+}
+
+template <int HT, bool FROM_LEAVES>
+__global__ void __launch_bounds__(kMerkleThreads) k_merkle_tile(MerkleArgs a)
+{
+    __shared__ uint64_t lds[kMerkleThreads][4];
+    const uint32_t t = threadIdx.x;
+    uint32_t cur = 1u << a.tile_log;     // nodes of this tile at the current level
+    uint32_t lg_w = a.lg_w;              // log2 per-subtree width of the current level
+    uint64_t first = (uint64_t)blockIdx.x << a.tile_log;  // first node of the tile
+
+    uint64_t h[4];
+    if (t < cur) {
+        const uint64_t g = first + t;
+        const uint64_t sub = g >> lg_w, idx = g & ((1ull << lg_w) - 1);
+        if constexpr (FROM_LEAVES) {
+            const uint64_t *row = a.leaves + g * a.leaf_stride;
+            const uint64_t cs = a.col_stride;
+            hash_one<HT>(a.leaf_size, [&](uint32_t k) { return row[(uint64_t)k * cs]; }, h);
+            if (lg_w >= 1) {
+                uint64_t *dst = node_ptr(a, sub, lg_w, idx);
+#pragma unroll
+                for (int i = 0; i < 4; i++) dst[i] = h[i];
+            }
+        } else {
+            const uint64_t *src = node_ptr(a, sub, lg_w, idx);
+#pragma unroll
+            for (int i = 0; i < 4; i++) h[i] = src[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) lds[t][i] = h[i];
+    }
+    __syncthreads();
+
+    while (true) {
+        if (lg_w == 1) {
+            // width-2 top level: caps of the subtrees in this tile
+            const uint32_t pairs = cur >> 1;
+            if (t < pairs) {
+                uint64_t out[4];
+                hash_two<HT>(lds[2 * t], lds[2 * t + 1], out);
+                const uint64_t sub = (first >> 1) + t;
+#pragma unroll
+                for (int i = 0; i < 4; i++) a.cap[4 * sub + i] = out[i];
+            }
+            return;
+        }
+        if (cur == 1) return;  // tile root: next pass continues from here
+        const uint32_t half = cur >> 1;
+        uint64_t out[4];
+        if (t < half) hash_two<HT>(lds[2 * t], lds[2 * t + 1], out);
+        __syncthreads();
+        lg_w -= 1;
+        first >>= 1;
+        if (t < half) {
+            const uint64_t g = first + t;
+            const uint64_t sub = g >> lg_w, idx = g & ((1ull << lg_w) - 1);
+            uint64_t *dst = node_ptr(a, sub, lg_w, idx);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                dst[i] = out[i];
+                lds[t][i] = out[i];
+            }
+        }
+        cur = half;
+        __syncthreads();
+    }
+}
+
+// caps == leaves: cap[i] = hash_one(leaf i)  (merkle.cu:23-38 direct case).
+template <int HT>
+__global__ void __launch_bounds__(kMerkleThreads) k_leaf_direct(MerkleArgs a)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.count) return;
+    const uint64_t *row = a.leaves + g * a.leaf_stride;
+    const uint64_t cs = a.col_stride;
+    uint64_t h[4];
+    hash_one<HT>(a.leaf_size, [&](uint32_t k) { return row[(uint64_t)k * cs]; }, h);
+#pragma unroll
+    for (int i = 0; i < 4; i++) a.cap[4 * g + i] = h[i];
+}
+
+}  // namespace zkx

@@ -1,0 +1,28 @@
+// Host-side NTT / LDE / transpose drivers (device pointers, enqueued on the
+// device's stream; the caller synchronises).
+#pragma once
+#include "runtime.h"
+
+namespace zkx {
+
+// LDE of `cols` columns of 2^lg_n coefficients into `out` (cols x 2^(lg_n+rate)),
+// each output column in bit-reversed order (DIF order, no reversal pass).
+void lde_bitrev(Device &d, uint64_t *out, const uint64_t *in, uint32_t lg_n, uint32_t rate, uint32_t cols,
+                bool inverse, bool with_coset);
+// compute_batched_lde semantics: natural-order output.
+void lde(Device &d, uint64_t *out, const uint64_t *in, uint32_t lg_n, uint32_t rate, uint32_t cols,
+         bool inverse, bool with_coset);
+// compute_batched_ntt semantics: in place, natural order.
+void ntt(Device &d, uint64_t *inout, uint32_t lg, uint32_t cols, bool inverse, bool with_coset);
+// In-place bit reversal of each column with optional post-scaling.
+void bitrev_columns(Device &d, uint64_t *buf, uint32_t lg, uint32_t cols, bool use_scale, uint64_t scale,
+                    const uint64_t *coset);
+// out[t * cols + b] = in[b * 2^lg + (rev ? bitrev(t) : t)].
+void transpose_rev(Device &d, uint64_t *out, const uint64_t *in, uint32_t lg, uint32_t cols, bool rev);
+
+// table[i] = base^i, i < n.
+void fill_powers(uint64_t *dst, uint64_t n, uint64_t base, hipStream_t s);
+// `count` pseudo-random field elements from `seed` (LDE salt columns).
+void fill_salt(uint64_t *dst, uint64_t count, uint64_t seed, hipStream_t s);
+
+}  // namespace zkx

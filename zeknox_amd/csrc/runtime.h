@@ -1,0 +1,101 @@
+// Device registry, streams, error propagation and per-device resident tables.
+//
+// Replaces the reference's utils/all_gpus.{hpp,cu} (device registry),
+// utils/gpu_t.cuh (streams, copies, temporary buffers) and
+// utils/exception.cuh (CUDA_OK -> thrown error, mapped to RustError by the
+// entry points) with HIP-native equivalents.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace zkx {
+
+// Thrown inside the library, converted to RustError{code, message} at the
+// C boundary.  code 1 = invalid argument; -e = HIP error e.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void throw_hip(hipError_t e, const char *what, const char *file, int line);
+[[noreturn]] inline void invalid(const std::string &m) { throw Error(1, m); }
+
+#define ZKX_HIP(call)                                                         \
+    do {                                                                      \
+        hipError_t zkx_e_ = (call);                                           \
+        if (zkx_e_ != hipSuccess) ::zkx::throw_hip(zkx_e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+// A device buffer owned by the library; freed on destruction.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n);
+    ~DevBuf();
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept
+    {
+        if (this != &o) {
+            if (p) (void)hipFree(p);
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct Device {
+    int id = 0;
+    hipStream_t stream = nullptr;  // blocking stream: ordered after the null stream
+    hipDeviceProp_t prop{};
+    std::mutex mu;                 // serialises table builds and compute calls
+
+    // Resident tables (twiddles per log size, coset powers).
+    uint64_t *twiddles[33] = {};   // twiddles[k][i] = w_{2^k}^i, i < 2^k
+    uint64_t *coset = nullptr;     // coset[i] = g^i, i < 2^coset_lg
+    uint32_t coset_lg = 0;
+    uint64_t coset_gen = 0;
+
+    const uint64_t *twiddle_table(uint32_t lg);  // builds on first use
+    void build_coset(uint32_t lg, uint64_t gen);
+    void sync();
+};
+
+int device_count();
+// Makes the device current and returns it; id is taken modulo 2^32 (Rust
+// callers pass i32 in a size_t slot).
+Device &device(size_t id);
+
+// Primitive 2^k-th root of unity (gl64_params.hpp OMEGA[k]).
+uint64_t omega(uint32_t k);
+uint64_t host_mul(uint64_t a, uint64_t b);
+uint64_t host_pow(uint64_t b, uint64_t e);
+uint64_t host_inv(uint64_t a);
+
+// Last error recorded by a void entry point (thread-local).
+void set_last_error(const std::string &m);
+const char *last_error();
+
+// Optional per-kernel timing: a ProfScope around a launch records HIP events
+// on the launching stream when profiling is enabled (zkx_profile_enable).
+bool profiling();
+struct ProfScope {
+    hipEvent_t e0 = nullptr;
+    const char *name;
+    hipStream_t stream;
+    ProfScope(const char *kernel_family, hipStream_t s);
+    ~ProfScope();
+};
+
+}  // namespace zkx
