@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""LDE + Merkle commit benchmark on one MI355X.
+
+Workload (one "step" = one commit): 135 polynomials of degree 2^15 with
+coefficients uniform in [0, p), rate_bits 3 -> 2^18-point coset LDE (shift
+7), bit-reversed rows transposed to 2^18 leaves of 135 elements, Poseidon
+Merkle tree with cap height 4.  Inputs are synthetic (seeded splitmix64,
+rejection-sampled into [0, p)) and resident in HBM before timing starts.
+
+Two paths are timed:
+  plain   -- the C-ABI sequence compute_batched_lde -> compute_transpose_rev
+             -> fill_digests_buf_linear_gpu_with_gpu_ptr, one commit per call;
+  batched -- the batched commit engine (zeknox_amd.batch), commits of the
+             same shape in flight together.
+The headline value is the batched rate; both, the per-kernel HIP-event
+timings, the roofline of the dominant kernel and a CPU baseline (the C
+oracle on host cores, bounded sample) go into one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# config-2 shape
+LG_N, RATE, COLS, CAP_H = 15, 3, 135, 4
+LG = LG_N + RATE
+N = 1 << LG
+HASH = 0  # Poseidon
+
+# MI355X published peaks
+HBM_PEAK_GBS = 8000.0
+
+# VALU wave-instructions of one config-2 commit's leaf-hash launch
+# (rocprofv3 --pmc SQ_INSTS_VALU; profiles/merkle_leaves_valu_config2.txt)
+VALU_WAVE_INSTS_PER_COMMIT = {0: 2.13e9, 3: 1.6e9}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--jobs", type=int, default=4, help="config-2 commits per batched step")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
+    ap.add_argument("--cpu-rows", type=int, default=1 << 13,
+                    help="leaf rows of the bounded CPU sample (of 2^18)")
+    return ap.parse_args()
+
+
+def perms_per_commit(rows, cols, caps):
+    # leaf sponge: ceil(cols/8) permutations per leaf (cols > 4), plus one per
+    # internal node and cap: rows - caps
+    per_leaf = (cols + 7) // 8 if cols > 4 else 0
+    return rows * per_leaf + (rows - caps)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build drives one GPU; --gpus {args.gpus} is not supported "
+              "(jobs would shard across GPUs by commit, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import zeknox_amd as zk
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+    ngpu = zk.get_number_of_gpus_rs()
+    assert ngpu >= 1, "no HIP device"
+    zk.init_coset_rs(0, LG, 7)
+    zk.init_twiddle_factors_rs(0, LG)
+
+    # synthetic inputs, resident in HBM
+    rng = np.random.default_rng(2024)
+    coeffs = rng.integers(0, 2**63, size=COLS << LG_N, dtype=np.uint64) * np.uint64(2)
+    coeffs %= np.uint64(zk.P)
+    d_in = zk.DeviceBuffer.from_host(coeffs)
+    d_lde = zk.DeviceBuffer(COLS * N)
+    d_leaves = zk.DeviceBuffer(COLS * N)
+    nc = 1 << CAP_H
+    nd = 2 * (N - nc)
+    d_dig = zk.DeviceBuffer(nd * 4)
+    d_cap = zk.DeviceBuffer(nc * 4)
+    cfg = zk.NTTConfig(batches=COLS, extension_rate_bits=RATE, with_coset=True, are_inputs_on_device=True,
+                       are_outputs_on_device=True)
+    tcfg = zk.TransposeConfig(batches=COLS, are_inputs_on_device=True, are_outputs_on_device=True)
+
+    def plain_commit():
+        zk.lde_batch(0, d_lde, d_in, LG_N, cfg)
+        zk.transpose_rev_batch(0, d_leaves, d_lde, LG, tcfg)
+        zk.fill_digests_buf_linear_gpu_with_gpu_ptr(d_dig, d_cap, d_leaves, nd, nc, N, COLS, CAP_H, HASH, 0)
+
+    # ---- plain path
+    for _ in range(args.warmup):
+        plain_commit()
+    zk.lib.zkx_device_synchronize(0)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plain_commit()
+    zk.lib.zkx_device_synchronize(0)
+    plain_s = (time.perf_counter() - t0) / args.steps
+
+    # per-kernel HIP-event timings of the same sequence (separate pass)
+    zk.kernel_stats_reset()
+    zk.profile_enable(True)
+    for _ in range(args.steps):
+        plain_commit()
+    stats = zk.kernel_stats()
+    zk.profile_enable(False)
+    kernels = {k: {"launches_per_commit": n / args.steps, "ms_per_commit": ms / args.steps,
+                   "avg_launch_ms": ms / n} for k, (n, ms) in stats.items()}
+
+    # ---- batched path: JOBS config-2 commits per zkx_commit_batch call
+    commits = [zk.Commit(d_in, COLS, LG_N, RATE, CAP_H, HASH) for _ in range(args.jobs)]
+    for _ in range(args.warmup):
+        zk.commit_batch(commits)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        zk.commit_batch(commits)
+    batch_step_s = (time.perf_counter() - t0) / args.steps
+    batched_s = batch_step_s / args.jobs  # seconds per commit
+    zk.kernel_stats_reset()
+    zk.profile_enable(True)
+    for _ in range(args.steps):
+        zk.commit_batch(commits)
+    bstats = zk.kernel_stats()
+    zk.profile_enable(False)
+    batch_kernels = {k: {"launches_per_step": n / args.steps, "ms_per_step": ms / args.steps, "avg_launch_ms": ms / n}
+                     for k, (n, ms) in bstats.items()}
+    # the batched outputs equal the plain path's (bit-exact) on this input
+    pl = d_leaves.to_host(COLS * 64), d_dig.to_host(), d_cap.to_host()
+    bl = commits[-1].results()
+    bit_exact = bool(np.array_equal(pl[0], bl[0][:COLS * 64]) and np.array_equal(pl[1], bl[1])
+                     and np.array_equal(pl[2], bl[2]))
+    mode = "batched"
+
+    # ---- roofline of the dominant kernel: the leaf-hash launch of the batch
+    # (Poseidon permutations, VALU-bound: 64-bit modular arithmetic on the
+    # 32-bit integer VALU, no MFMA).  achieved = VALU issue slots per launch
+    # (SQ_INSTS_VALU of that launch x 64 lanes, measured with rocprofv3 --pmc
+    # on this workload: profiles/merkle_leaves_valu_config2.txt) / the
+    # launch's HIP-event duration; peak = 256 CUs x 64 lanes x 2.4 GHz.
+    dom = max(batch_kernels, key=lambda k: batch_kernels[k]["ms_per_step"])
+    leaf = batch_kernels.get("merkle_leaves")
+    valu_peak = 256 * 64 * 2.4e9 / 1e12  # TOP/s
+    roofline = None
+    if leaf:
+        insts = VALU_WAVE_INSTS_PER_COMMIT[HASH] * args.jobs
+        achieved = insts * 64 / (leaf["avg_launch_ms"] * 1e-3) / 1e12
+        roofline = {"bound": "valu", "achieved": round(achieved, 2), "peak": round(valu_peak, 2),
+                    "unit": "TOP/s (32-bit VALU lane-ops)", "frac": round(achieved / valu_peak, 3),
+                    "traffic": None, "kernel": "merkle_leaves",
+                    "avg_launch_ms": round(leaf["avg_launch_ms"], 4),
+                    "perms_per_launch": args.jobs * perms_per_commit(N, COLS, nc)}
+
+    # LDE passes against HBM: algorithmic bytes of the LDE (read n, write N
+    # per column) over the batch's NTT launches
+    ntt = batch_kernels.get("batch_ntt")
+    ntt_roofline = None
+    if ntt:
+        lde_bytes = 8 * COLS * ((1 << LG_N) + N) * args.jobs
+        gbs = lde_bytes / (ntt["ms_per_step"] * 1e-3) / 1e9
+        ntt_roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(gbs / HBM_PEAK_GBS, 3), "traffic": None, "kernel": "batch_ntt (all passes)"}
+
+    # ---- CPU baseline: the C oracle on host cores, bounded sample
+    cpu = None
+    if not args.no_cpu:
+        import oracle_bind as ob
+        rows = args.cpu_rows
+        leaves = ob.canonical(rows * COLS, 7)
+        t = time.perf_counter()
+        ob.merkle(leaves, COLS, CAP_H, HASH)
+        merkle_s = time.perf_counter() - t
+        x = ob.canonical(COLS << LG_N, 9)
+        t = time.perf_counter()
+        lde = ob.lde(x, LG_N, RATE, COLS)
+        ob.transpose_rev(lde, LG, COLS)
+        lde_s = time.perf_counter() - t
+        commit_s = lde_s + merkle_s * (N / rows)
+        cpu = {"value": round(1.0 / commit_s, 4), "unit": "commits/s", "cores": ob.orc.orc_num_threads(),
+               "kind": "port",
+               "sample": f"full LDE+transpose of 135x2^15 -> 2^18 ({lde_s:.2f} s) + Poseidon Merkle over "
+                         f"{rows} of 2^18 leaf rows ({merkle_s:.2f} s), scaled to one commit"}
+
+    value = args.jobs / batch_step_s
+    out = {
+        "metric": "LDE+Merkle commits/s batched vs plain at bit-exact; p50/p99 commit latency",
+        "value": round(value, 2),
+        "unit": "commits/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(batch_step_s * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 (Goldilocks field)",
+        "data": "synthetic (seeded uniform canonical field elements)",
+        "config": {"workload": "LDE+commit: 135 polys x 2^15, rate_bits 3 (2^18-point coset LDE, shift 7), "
+                               "transpose to 2^18 leaves x 135, Poseidon Merkle, cap_height 4",
+                   "path": mode, "commits_per_step": args.jobs, "parallelism": "single GPU"},
+        "plain_commits_per_s": round(1.0 / plain_s, 2),
+        "batched_over_plain": round(plain_s / batched_s, 3),
+        "batched_equals_plain": bit_exact,
+        "kernels_plain": kernels,
+        "kernels_batched": batch_kernels,
+        "dominant_kernel": dom,
+        "roofline": roofline,
+        "ntt_roofline": ntt_roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -31,6 +31,29 @@ RustError zkx_device_synchronize(size_t device_id);
 /* The stream the library enqueues its work on (a hipStream_t). */
 void *zkx_stream(size_t device_id);
 
+/* ---- batched commits ---------------------------------------------------
+ * One commit = coset LDE (generator of init_coset) of `cols` polynomials of
+ * 2^lg_n coefficients (column-major, device) with blowup 2^rate; the leaf
+ * rows leaves[t * cols + b] = LDE_b[bitrev(t)] (compute_transpose_rev of the
+ * LDE; optional, NULL to skip); the Merkle forest of those rows with
+ * 2^cap_height caps (hash_type 0 Poseidon, 3 Poseidon2).  Outputs are
+ * bit-identical to compute_batched_lde -> compute_transpose_rev ->
+ * fill_digests_buf_linear_gpu_with_gpu_ptr on the same inputs.  All pointers
+ * are device pointers; sizes as in merkle.h. */
+typedef struct {
+    const void *coeffs;
+    void *leaves;
+    void *digests;
+    void *caps;
+    uint32_t cols, lg_n, rate, cap_height, hash_type, reserved;
+} ZkxCommitJob;
+
+/* All jobs of one call run together (one launch per pass over all jobs);
+ * returns when they are complete. */
+RustError zkx_commit_batch(size_t device_id, const ZkxCommitJob *jobs, size_t njobs);
+/* Device working memory zkx_commit_batch needs for these jobs, in bytes. */
+size_t zkx_commit_scratch_bytes(const ZkxCommitJob *jobs, size_t njobs);
+
 /* Kernel timing: when enabled, every launch of the library's kernels on a
  * device is bracketed by HIP events on the launching stream; zkx_kernel_stats
  * reports, per kernel family, the launch count and summed duration. */

@@ -68,6 +68,22 @@ class TransposeConfig(ctypes.Structure):
         super().__init__(batches, are_inputs_on_device, are_outputs_on_device)
 
 
+class CommitJob(ctypes.Structure):
+    """zeknox_ext.h ZkxCommitJob (device pointers)."""
+    _fields_ = [
+        ("coeffs", ctypes.c_void_p),
+        ("leaves", ctypes.c_void_p),
+        ("digests", ctypes.c_void_p),
+        ("caps", ctypes.c_void_p),
+        ("cols", ctypes.c_uint32),
+        ("lg_n", ctypes.c_uint32),
+        ("rate", ctypes.c_uint32),
+        ("cap_height", ctypes.c_uint32),
+        ("hash_type", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run zeknox_amd.build.build() (hipcc, gfx950)")
@@ -100,6 +116,8 @@ def _load():
         "zkx_profile_enable": (RustError, [ctypes.c_int]),
         "zkx_kernel_stats": (ctypes.c_int, [vp, vp, vp, ctypes.c_int]),
         "zkx_kernel_stats_reset": (None, []),
+        "zkx_commit_batch": (RustError, [sz, ctypes.POINTER(CommitJob), sz]),
+        "zkx_commit_scratch_bytes": (sz, [ctypes.POINTER(CommitJob), sz]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -258,6 +276,44 @@ def merkle_tree(leaves, leaf_size, cap_height, hash_type=HASH_POSEIDON, device_i
     dd = DeviceBuffer(max(nd, 1) * 4, device_id)
     fill_digests_buf_linear_gpu_with_gpu_ptr(dd, dc, dl, nd, nc, nl, leaf_size, cap_height, hash_type, device_id)
     return (dd.to_host(nd * 4) if nd else np.zeros(0, np.uint64)), dc.to_host()
+
+
+# ---- batched commits --------------------------------------------------------
+
+class Commit:
+    """Device buffers of one commit job: coefficients in, leaf rows, digests
+    and caps out (zeknox_ext.h ZkxCommitJob)."""
+
+    def __init__(self, coeffs, cols, lg_n, rate, cap_height, hash_type=HASH_POSEIDON, leaves=True, device_id=0):
+        self.cols, self.lg_n, self.rate = cols, lg_n, rate
+        self.cap_height, self.hash_type = cap_height, hash_type
+        self.N = 1 << (lg_n + rate)
+        self.num_caps = 1 << cap_height
+        self.num_digests = 0 if self.num_caps == self.N else 2 * (self.N - self.num_caps)
+        self.d_coeffs = coeffs if isinstance(coeffs, DeviceBuffer) else DeviceBuffer.from_host(coeffs, device_id)
+        self.d_leaves = DeviceBuffer(cols * self.N, device_id) if leaves else None
+        self.d_digests = DeviceBuffer(max(self.num_digests, 1) * 4, device_id)
+        self.d_caps = DeviceBuffer(self.num_caps * 4, device_id)
+
+    def job(self):
+        return CommitJob(self.d_coeffs.ptr, self.d_leaves.ptr if self.d_leaves else None, self.d_digests.ptr,
+                         self.d_caps.ptr, self.cols, self.lg_n, self.rate, self.cap_height, self.hash_type, 0)
+
+    def results(self):
+        leaves = self.d_leaves.to_host() if self.d_leaves else None
+        dig = self.d_digests.to_host(self.num_digests * 4) if self.num_digests else np.zeros(0, np.uint64)
+        return leaves, dig, self.d_caps.to_host()
+
+
+def commit_batch(commits, device_id=0):
+    """Run a list of Commit objects as one batch (synchronous)."""
+    arr = (CommitJob * len(commits))(*[c.job() for c in commits])
+    _check(lib.zkx_commit_batch(device_id, arr, len(commits)))
+
+
+def commit_scratch_bytes(commits):
+    arr = (CommitJob * len(commits))(*[c.job() for c in commits])
+    return lib.zkx_commit_scratch_bytes(arr, len(commits))
 
 
 # ---- timing ---------------------------------------------------------------

@@ -1,0 +1,72 @@
+// Batched LDE + Merkle commit engine.
+//
+// A commit job = coset LDE of `cols` polynomials of 2^lg_n coefficients with
+// blowup 2^rate, its leaf matrix (row t = the evaluations at bitrev(t), i.e.
+// compute_transpose_rev of the LDE), and the Poseidon / Poseidon2 Merkle
+// forest with 2^cap_height caps over those rows -- exactly what the C-ABI
+// sequence compute_batched_lde -> compute_transpose_rev ->
+// fill_digests_buf_linear_gpu_with_gpu_ptr produces.
+//
+// An Engine runs any number of uneven jobs together on one stream: one
+// launch per NTT pass index covers that pass of every job, one launch per
+// Merkle pass index and hash kind covers that pass of every tree, so small
+// jobs share the GPU and the latency-bound top levels of different trees run
+// side by side.  The LDE stays in HBM in its natural DIF output order
+// (bit-reversed, column-major) and the leaf kernels read it there directly:
+// no bit-reversal or transpose pass; the leaf rows are written by the leaf
+// kernel as it reads them.
+#pragma once
+#include <vector>
+
+#include "merkle.h"
+#include "ntt.h"
+#include "runtime.h"
+
+namespace zkx {
+
+struct CommitJob {
+    const uint64_t *coeffs = nullptr;  // device, cols x 2^lg_n, column-major
+    uint64_t *leaves = nullptr;        // device, 2^(lg_n+rate) x cols rows, or null
+    uint64_t *digests = nullptr;       // device, 2 * (N - 2^cap_height) x 4
+    uint64_t *caps = nullptr;          // device, 2^cap_height x 4
+    uint32_t cols = 0, lg_n = 0, rate = 0, cap_height = 0, hash_type = 0;
+};
+
+void check_commit_job(const CommitJob &j);
+// Device bytes of working memory an Engine needs for these jobs.
+size_t commit_scratch_bytes(const CommitJob *jobs, size_t n);
+
+class Engine {
+public:
+    // stream == nullptr: the engine creates its own (default-priority) stream
+    Engine(Device &d, hipStream_t stream = nullptr);
+    ~Engine();
+    Engine(const Engine &) = delete;
+    Engine &operator=(const Engine &) = delete;
+
+    // Enqueue the jobs on the engine's stream; returns without waiting.  The
+    // tables the jobs need are built on first use.  Calls on one engine are
+    // serialised by `mu`.
+    void commit(const CommitJob *jobs, size_t n);
+    std::mutex mu;
+    void sync();
+    hipStream_t stream() const { return stream_; }
+    Device &device() { return dev_; }
+
+private:
+    void stage(const void *src, size_t bytes, size_t &offset);
+
+    Device &dev_;
+    hipStream_t stream_ = nullptr;
+    bool own_stream_ = false;
+    DevBuf scratch_;           // LDE of every job, DIF order
+    DevBuf desc_;              // descriptor lists (device)
+    void *pinned_ = nullptr;   // descriptor staging (host, pinned)
+    size_t pinned_bytes_ = 0;
+    hipEvent_t staged_ = nullptr;  // last staging copy
+};
+
+// The device's engine on the device stream (used by zkx_commit_batch).
+Engine &default_engine(Device &d);
+
+}  // namespace zkx

@@ -1,0 +1,223 @@
+// Batched LDE + Merkle commit engine (see batch.h).
+#include "batch.h"
+
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <string>
+
+namespace zkx {
+
+namespace {
+
+constexpr size_t kAlign = 256;
+
+size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
+
+uint64_t lde_elems(const CommitJob &j) { return (uint64_t)j.cols << (j.lg_n + j.rate); }
+
+}  // namespace
+
+void check_commit_job(const CommitJob &j)
+{
+    if (j.hash_type != 0 && j.hash_type != 3) invalid("commit job: hash type must be 0 (Poseidon) or 3 (Poseidon2)");
+    if (j.cols == 0) invalid("commit job: no columns");
+    if (j.lg_n == 0 || j.lg_n + j.rate > 30) invalid("commit job: log degree out of range");
+    if (j.rate < 1 || j.rate > kMaxRateBits) invalid("commit job: rate_bits must be in [1, 5]");
+    if (j.cap_height > j.lg_n + j.rate) invalid("commit job: cap height above the tree height");
+    if (!j.coeffs || !j.caps) invalid("commit job: null coefficient or cap buffer");
+    if (j.cap_height < j.lg_n + j.rate && !j.digests) invalid("commit job: null digest buffer");
+}
+
+size_t commit_scratch_bytes(const CommitJob *jobs, size_t n)
+{
+    size_t s = 0;
+    for (size_t i = 0; i < n; i++) s += align_up(lde_elems(jobs[i]) * sizeof(uint64_t));
+    return s;
+}
+
+Engine::Engine(Device &d, hipStream_t stream) : dev_(d), stream_(stream)
+{
+    ZKX_HIP(hipSetDevice(d.id));
+    if (!stream_) {
+        ZKX_HIP(hipStreamCreate(&stream_));
+        own_stream_ = true;
+    }
+    ZKX_HIP(hipEventCreateWithFlags(&staged_, hipEventDisableTiming));
+}
+
+Engine::~Engine()
+{
+    (void)hipSetDevice(dev_.id);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (staged_) (void)hipEventDestroy(staged_);
+    if (pinned_) (void)hipHostFree(pinned_);
+    if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Engine::sync() { ZKX_HIP(hipStreamSynchronize(stream_)); }
+
+void Engine::commit(const CommitJob *jobs, size_t n)
+{
+    if (n == 0) return;
+    for (size_t i = 0; i < n; i++) check_commit_job(jobs[i]);
+    ZKX_HIP(hipSetDevice(dev_.id));
+
+    // ---- working memory: every job's LDE in DIF (bit-reversed) order
+    const size_t need = commit_scratch_bytes(jobs, n);
+    if (scratch_.bytes < need) {
+        sync();
+        scratch_ = DevBuf(need);
+    }
+    std::vector<uint64_t *> lde(n);
+    {
+        size_t off = 0;
+        for (size_t i = 0; i < n; i++) {
+            lde[i] = reinterpret_cast<uint64_t *>(static_cast<char *>(scratch_.p) + off);
+            off += align_up(lde_elems(jobs[i]) * sizeof(uint64_t));
+        }
+    }
+
+    // ---- resident tables (built on the device stream, which this engine's
+    // stream is ordered after: the null-stream-blocking device stream is
+    // synchronised before first use)
+    std::vector<const uint64_t *> tw(n), F(n);
+    {
+        std::lock_guard<std::mutex> g(dev_.mu);
+        bool built = false;
+        for (size_t i = 0; i < n; i++) {
+            const uint32_t lg = jobs[i].lg_n + jobs[i].rate;
+            built |= dev_.twiddles[lg] == nullptr;
+            tw[i] = dev_.twiddle_table(lg);
+            const size_t before = dev_.lde_factors.size();
+            F[i] = lde_factor_table(dev_, jobs[i].lg_n, jobs[i].rate, false, true);
+            built |= dev_.lde_factors.size() != before;
+        }
+        if (built) dev_.sync();
+    }
+
+    // ---- plans
+    std::vector<std::vector<NttPass>> ntt_plan(n);
+    std::vector<uint64_t> ntt_tiles(n);
+    std::vector<std::vector<MerkleArgs>> mk_plan(n);
+    size_t max_ntt = 0, max_mk = 0;
+    for (size_t i = 0; i < n; i++) {
+        const CommitJob &j = jobs[i];
+        const uint32_t lg = j.lg_n + j.rate;
+        ntt_tiles[i] = plan_dif(ntt_plan[i], lde[i], (uint64_t)j.cols << j.rate, j.lg_n, lg, tw[i], false, kSrcLde,
+                                j.coeffs, j.rate, false, F[i]);
+        MerkleSpec s;
+        s.digests = j.digests;
+        s.caps = j.caps;
+        s.leaves = lde[i];
+        s.leaves_out = j.leaves;
+        s.num_leaves = 1ull << lg;
+        s.num_caps = 1ull << j.cap_height;
+        s.num_digests = s.num_caps == s.num_leaves ? 0 : 2 * (s.num_leaves - s.num_caps);
+        s.leaf_size = j.cols;
+        s.cap_height = j.cap_height;
+        s.hash_type = j.hash_type;
+        s.leaf_stride = 1;
+        s.col_stride = 1ull << lg;
+        check_merkle(s);
+        plan_merkle(s, mk_plan[i]);
+        max_ntt = std::max(max_ntt, ntt_plan[i].size());
+        max_mk = std::max(max_mk, mk_plan[i].size());
+    }
+
+    // ---- descriptor lists: NTT pass k of every job; Merkle pass k of every
+    // tree of each hash kind.  All staged with one host->device copy.
+    struct Launch {
+        size_t offset;  // bytes into desc_
+        uint32_t count;
+        uint64_t blocks;
+        int kind;       // 0 NTT, 1 Merkle
+        int hash;
+        bool leaves;
+    };
+    std::vector<Launch> launches;
+    std::vector<char> host;
+    auto push = [&](const void *p, size_t bytes) {
+        const size_t off = host.size();
+        host.resize(off + bytes);
+        memcpy(host.data() + off, p, bytes);
+        return off;
+    };
+    for (size_t k = 0; k < max_ntt; k++) {
+        std::vector<NttPass> list;
+        uint64_t blocks = 0;
+        for (size_t i = 0; i < n; i++) {
+            if (k >= ntt_plan[i].size()) continue;
+            NttPass p = ntt_plan[i][k];
+            p.tile_begin = blocks;
+            blocks += ntt_tiles[i];
+            list.push_back(p);
+        }
+        const size_t off = push(list.data(), list.size() * sizeof(NttPass));
+        launches.push_back({off, (uint32_t)list.size(), blocks, 0, 0, false});
+        host.resize(align_up(host.size()));
+    }
+    for (size_t k = 0; k < max_mk; k++)
+        for (int ht : {0, 3}) {
+            std::vector<MerkleArgs> list;
+            uint64_t blocks = 0;
+            for (size_t i = 0; i < n; i++) {
+                if ((int)jobs[i].hash_type != ht || k >= mk_plan[i].size()) continue;
+                MerkleArgs a = mk_plan[i][k];
+                a.tile_begin = blocks;
+                blocks += a.count >> a.tile_log;
+                list.push_back(a);
+            }
+            if (list.empty()) continue;
+            const size_t off = push(list.data(), list.size() * sizeof(MerkleArgs));
+            launches.push_back({off, (uint32_t)list.size(), blocks, 1, ht, k == 0});
+            host.resize(align_up(host.size()));
+        }
+    size_t off0 = 0;
+    stage(host.data(), host.size(), off0);
+    (void)off0;
+
+    // ---- launches, in dependency order on one stream
+    for (const Launch &l : launches) {
+        const char *base = static_cast<const char *>(desc_.p) + l.offset;
+        if (l.kind == 0) {
+            launch_ntt_list(stream_, reinterpret_cast<const NttPass *>(base), l.count, l.blocks, "batch_ntt");
+        } else {
+            launch_merkle_pass(stream_, l.hash, l.leaves, reinterpret_cast<const MerkleArgs *>(base), l.count,
+                               MerkleArgs{}, l.blocks);
+        }
+    }
+}
+
+Engine &default_engine(Device &d)
+{
+    static std::mutex mu;
+    static std::map<int, std::unique_ptr<Engine>> engines;
+    std::lock_guard<std::mutex> g(mu);
+    auto &e = engines[d.id];
+    if (!e) e = std::make_unique<Engine>(d, d.stream);
+    return *e;
+}
+
+void Engine::stage(const void *src, size_t bytes, size_t &offset)
+{
+    // the previous staging copy must have left the pinned buffer
+    ZKX_HIP(hipEventSynchronize(staged_));
+    if (pinned_bytes_ < bytes) {
+        if (pinned_) ZKX_HIP(hipHostFree(pinned_));
+        pinned_bytes_ = std::max(bytes, (size_t)1 << 16);
+        ZKX_HIP(hipHostMalloc(&pinned_, pinned_bytes_, hipHostMallocDefault));
+    }
+    if (desc_.bytes < bytes) {
+        sync();
+        desc_ = DevBuf(std::max(bytes, (size_t)1 << 16));
+    }
+    memcpy(pinned_, src, bytes);
+    ZKX_HIP(hipMemcpyAsync(desc_.p, pinned_, bytes, hipMemcpyHostToDevice, stream_));
+    ZKX_HIP(hipEventRecord(staged_, stream_));
+    offset = 0;
+}
+
+}  // namespace zkx

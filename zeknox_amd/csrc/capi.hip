@@ -7,6 +7,7 @@
 #include <time.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/zeknox.h"
 #include "merkle.h"
@@ -270,6 +271,59 @@ void fill_digests_buf_linear_cpu(void *digests_buf_ptr, void *cap_buf_ptr, const
 {
     merkle_void(digests_buf_ptr, cap_buf_ptr, leaves_buf_ptr, digests_buf_size, cap_buf_size, leaves_buf_size,
                 leaf_size, cap_height, hash_type, 0, true);
+}
+
+}  // extern "C"
+
+// ---- batched commits (zeknox_ext.h) -----------------------------------------
+
+#include "../../include/zeknox_ext.h"
+#include "batch.h"
+
+namespace {
+
+std::vector<CommitJob> to_jobs(const ZkxCommitJob *jobs, size_t n)
+{
+    if (n && !jobs) invalid("null job array");
+    std::vector<CommitJob> v(n);
+    for (size_t i = 0; i < n; i++) {
+        v[i].coeffs = static_cast<const uint64_t *>(jobs[i].coeffs);
+        v[i].leaves = static_cast<uint64_t *>(jobs[i].leaves);
+        v[i].digests = static_cast<uint64_t *>(jobs[i].digests);
+        v[i].caps = static_cast<uint64_t *>(jobs[i].caps);
+        v[i].cols = jobs[i].cols;
+        v[i].lg_n = jobs[i].lg_n;
+        v[i].rate = jobs[i].rate;
+        v[i].cap_height = jobs[i].cap_height;
+        v[i].hash_type = jobs[i].hash_type;
+    }
+    return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+RustError zkx_commit_batch(size_t device_id, const ZkxCommitJob *jobs, size_t njobs)
+{
+    return guarded([&] {
+        std::vector<CommitJob> v = to_jobs(jobs, njobs);
+        Device &d = device(device_id);
+        Engine &e = default_engine(d);
+        std::lock_guard<std::mutex> g(e.mu);
+        e.commit(v.data(), v.size());
+        e.sync();
+    });
+}
+
+size_t zkx_commit_scratch_bytes(const ZkxCommitJob *jobs, size_t njobs)
+{
+    try {
+        std::vector<CommitJob> v = to_jobs(jobs, njobs);
+        return commit_scratch_bytes(v.data(), v.size());
+    } catch (...) {
+        return 0;
+    }
 }
 
 }  // extern "C"

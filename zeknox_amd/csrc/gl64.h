@@ -1,12 +1,19 @@
 // Goldilocks field p = 2^64 - 2^32 + 1 on the CDNA VALU.
 //
-// Values are u64 held in two VGPRs.  Every function here takes and returns
-// canonical values in [0, p), which is the representation the reference's
-// digests and LDE outputs are compared in (ff/gl64_t.cuh:485-506 reduces to
-// canonical; ff/goldilocks.hpp:139-177 keeps canonical values).
+// Values are u64 held in two VGPRs.  Every function takes and returns
+// canonical values in [0, p) -- the representation the reference's digests
+// and LDE outputs are compared in (ff/gl64_t.cuh:485-506 reduces to
+// canonical; ff/goldilocks.hpp:139-177 keeps canonical values) -- unless its
+// comment says otherwise.
 //
-// Multiplication is built from 32x32->64 pieces (v_mad_u64_u32) and reduced
-// with 2^64 = 2^32 - 1 (mod p) and 2^96 = -1 (mod p).
+// Instruction selection.  On gfx950 the 32-bit integer multiply-adds
+// (v_mad_u64_u32, v_mul_hi_u32) issue at full rate, while the 64-bit VALU
+// forms (v_lshl_add_u64, v_cmp_*_u64) run at about a third of it.  All
+// additions and comparisons here are therefore written as 32-bit carry
+// chains (__builtin_addc / __builtin_subc -> v_add_co_u32 / v_addc_co_u32,
+// carries in SGPR masks, results picked with v_cndmask_b32), and products
+// are built from four v_mad_u64_u32.  Reduction uses 2^64 = 2^32 - 1 and
+// 2^96 = -1 (mod p).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -16,54 +23,76 @@ namespace gl {
 constexpr uint64_t P = 0xFFFFFFFF00000001ull;
 constexpr uint64_t EPS = 0xFFFFFFFFull;  // 2^64 mod p
 
+__device__ __forceinline__ uint32_t lo32(uint64_t x) { return (uint32_t)x; }
+__device__ __forceinline__ uint32_t hi32(uint64_t x) { return (uint32_t)(x >> 32); }
+__device__ __forceinline__ uint64_t pack(uint32_t lo, uint32_t hi) { return ((uint64_t)hi << 32) | lo; }
+
+// (a + b) mod p for a < 2^64 and b < p, canonical when either
+//   a < p, or b <= 2^64 - 2^33 + 1 (then a may be any u64).
+// s = a + b; t = s + EPS = s - p (mod 2^64); a + b >= p iff either carries.
 __device__ __forceinline__ uint64_t add(uint64_t a, uint64_t b)
 {
-    uint64_t s = a + b;
-    // a + b >= 2^64 (carry) or p <= s < 2^64: subtract p, i.e. add EPS mod 2^64
-    return (s < a || s >= P) ? s + EPS : s;
+    uint32_t c1, c2, c3, c4;
+    const uint32_t slo = __builtin_addc(lo32(a), lo32(b), 0u, &c1);
+    const uint32_t shi = __builtin_addc(hi32(a), hi32(b), c1, &c2);
+    const uint32_t tlo = __builtin_addc(slo, 0xFFFFFFFFu, 0u, &c3);
+    const uint32_t thi = __builtin_addc(shi, 0u, c3, &c4);
+    return (c2 | c4) ? pack(tlo, thi) : pack(slo, shi);
 }
 
+// (a - b) mod p for a, b < p: borrow -> add p, i.e. subtract EPS.
 __device__ __forceinline__ uint64_t sub(uint64_t a, uint64_t b)
 {
-    uint64_t d = a - b;
-    return (a < b) ? d - EPS : d;  // borrow: d + p = d - EPS mod 2^64
+    uint32_t b1, b2, b3, b4;
+    const uint32_t dlo = __builtin_subc(lo32(a), lo32(b), 0u, &b1);
+    const uint32_t dhi = __builtin_subc(hi32(a), hi32(b), b1, &b2);
+    const uint32_t tlo = __builtin_subc(dlo, 0xFFFFFFFFu, 0u, &b3);
+    const uint32_t thi = __builtin_subc(dhi, 0u, b3, &b4);
+    return b2 ? pack(tlo, thi) : pack(dlo, dhi);
 }
 
-__device__ __forceinline__ uint64_t neg(uint64_t a) { return a ? P - a : 0; }
+__device__ __forceinline__ uint64_t neg(uint64_t a) { return sub(0, a); }
 
-// Reduce hi * 2^64 + lo (any 128-bit value) to [0, p).
+// h * EPS = h * 2^32 - h for h < 2^32 (exact, <= 2^64 - 2^33 + 1).
+__device__ __forceinline__ uint64_t mul_eps(uint32_t h)
+{
+    uint32_t b1, b2;
+    const uint32_t ylo = __builtin_subc(0u, h, 0u, &b1);
+    const uint32_t yhi = __builtin_subc(h, 0u, b1, &b2);
+    return pack(ylo, yhi);
+}
+
+// Canonical residue of hi * 2^64 + lo (any 128-bit value):
+//   hi * 2^64 = h0 * 2^64 + h1 * 2^96 = h0 * EPS - h1  (mod p).
 __device__ __forceinline__ uint64_t reduce128(uint64_t lo, uint64_t hi)
 {
-    uint32_t h0 = (uint32_t)hi, h1 = (uint32_t)(hi >> 32);
-    // hi*2^64 = h0*2^64 + h1*2^96 = h0*EPS - h1
-    uint64_t x = lo - h1;
-    if (lo < h1) x -= EPS;
-    uint64_t y = ((uint64_t)h0 << 32) - h0;  // h0 * EPS, exact
-    uint64_t z = x + y;
-    if (z < y) z += EPS;
-    return z >= P ? z - P : z;
+    const uint32_t h0 = lo32(hi), h1 = hi32(hi);
+    uint32_t b1, b2, b3, b4;
+    const uint32_t xlo = __builtin_subc(lo32(lo), h1, 0u, &b1);
+    const uint32_t xhi = __builtin_subc(hi32(lo), 0u, b1, &b2);
+    // borrow: x = lo - h1 + 2^64, take off 2^64 = EPS (no underflow: x >= 2^64 - 2^32 + 1)
+    const uint32_t elo = __builtin_subc(xlo, 0xFFFFFFFFu, 0u, &b3);
+    const uint32_t ehi = __builtin_subc(xhi, 0u, b3, &b4);
+    const uint64_t x = b2 ? pack(elo, ehi) : pack(xlo, xhi);
+    return add(x, mul_eps(h0));
 }
 
-// Reduce hi * 2^64 + lo with hi < 2^32.
-__device__ __forceinline__ uint64_t reduce96(uint64_t lo, uint32_t hi)
-{
-    uint64_t y = ((uint64_t)hi << 32) - hi;
-    uint64_t z = lo + y;
-    if (z < y) z += EPS;
-    return z >= P ? z - P : z;
-}
+// Canonical residue of top * 2^64 + lo with top < 2^32.
+__device__ __forceinline__ uint64_t reduce96(uint64_t lo, uint32_t top) { return add(lo, mul_eps(top)); }
 
-// 64x64 -> 128 product from four 32x32 multiply-adds.
+// 64x64 -> 128 product from four 32x32+64 multiply-adds.
 __device__ __forceinline__ void mul128(uint64_t a, uint64_t b, uint64_t &lo, uint64_t &hi)
 {
-    uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
-    uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
-    uint64_t t0 = (uint64_t)a0 * b0;
-    uint64_t t1 = (uint64_t)a0 * b1 + (t0 >> 32);
-    uint64_t t2 = (uint64_t)a1 * b0 + (uint32_t)t1;
-    uint64_t t3 = (uint64_t)a1 * b1 + (t1 >> 32);
-    hi = t3 + (t2 >> 32);
-    lo = (t2 << 32) | (uint32_t)t0;
+    const uint32_t a0 = lo32(a), a1 = hi32(a), b0 = lo32(b), b1 = hi32(b);
+    const uint64_t t0 = (uint64_t)a0 * b0;
+    const uint64_t t1 = (uint64_t)a1 * b0 + hi32(t0);
+    const uint64_t t2 = (uint64_t)a0 * b1 + lo32(t1);
+    const uint64_t t3 = (uint64_t)a1 * b1 + hi32(t1);  // < 2^64 - 2^32 + 1
+    uint32_t c1, c2;
+    const uint32_t hlo = __builtin_addc(lo32(t3), hi32(t2), 0u, &c1);
+    const uint32_t hhi = __builtin_addc(hi32(t3), 0u, c1, &c2);
+    hi = pack(hlo, hhi);
+    lo = pack(lo32(t0), lo32(t2));
 }
 
 __device__ __forceinline__ uint64_t mul(uint64_t a, uint64_t b)
@@ -75,11 +104,12 @@ __device__ __forceinline__ uint64_t mul(uint64_t a, uint64_t b)
 
 __device__ __forceinline__ uint64_t sqr(uint64_t a) { return mul(a, a); }
 
+// x^7 with the addition chain x2 = x^2, x3 = x*x2, x4 = x2^2, x7 = x3*x4.
 __device__ __forceinline__ uint64_t pow7(uint64_t x)
 {
-    uint64_t x2 = mul(x, x);
-    uint64_t x3 = mul(x2, x);
-    uint64_t x4 = mul(x2, x2);
+    const uint64_t x2 = mul(x, x);
+    const uint64_t x3 = mul(x2, x);
+    const uint64_t x4 = mul(x2, x2);
     return mul(x3, x4);
 }
 

@@ -1,8 +1,9 @@
 // Poseidon and Poseidon2 permutations (Goldilocks, width 12, rate 8,
 // 8 full + 22 partial rounds, x^7 S-box) and the plonky2 sponge, for one
-// lane: the 12-element state lives in VGPRs, round constants are
-// compile-time immediates after full unrolling (wave-uniform by
-// construction, no memory traffic).
+// lane.  The 12-element state lives in VGPRs.  Rounds run as a rolled loop
+// (the unrolled permutation is ~150 KB of code, more than the instruction
+// cache); round constants are read through the uniform round counter from
+// constant memory, i.e. as scalar loads into SGPRs.
 //
 // Poseidon      : poseidon/poseidon.cu:563-838 (naive rounds; same function
 //                 as the fast-partial-round form of poseidon/poseidon.cpp).
@@ -17,127 +18,158 @@ namespace zkx {
 
 enum HashKind : int { kPoseidon = 0, kPoseidon2 = 3 };
 
+// Round constants, [round][12].  The Poseidon table has one extra all-zero
+// row: after round r the next round's constants (row r+1) are folded into
+// the MDS accumulators, and the last round folds the zero row.
+static __constant__ uint64_t c_poseidon_rc[31 * 12] = {ZK_POSEIDON_RC_LIST 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+static __constant__ uint64_t c_poseidon2_rc[30 * 12] = {ZK_POSEIDON2_RC_LIST};
+
 // ---------------------------------------------------------------- Poseidon
 
-// state <- MDS * state, MDS = circ(17,15,41,16,2,28,13,13,39,18,34,20) +
-// diag(8,0,...).  Every coefficient is < 64 and a row's coefficients sum to
-// at most 264, so each element is split into 22/22/20-bit limbs and the 12x12
-// products are accumulated per limb in 32 bits with v_mad_u32_u24
-// (limb < 2^22, sum < 2^22 * 264 < 2^31).
-__device__ __forceinline__ void poseidon_mds(uint64_t s[12])
+// value = A + B * 2^32 (A, B < 2^64 - 2^32, value < 2^96), reduced: the
+// middle word is B + hi(A) formed with one v_mad_u64_u32 (x * 1 + y).
+__device__ __forceinline__ uint64_t limb_combine(uint64_t A, uint64_t B)
 {
-    uint32_t l0[12], l1[12], l2[12];
+    const uint64_t T = B + (uint64_t)gl::hi32(A);  // < 2^64
+    return gl::reduce96(gl::pack(gl::lo32(A), gl::lo32(T)), gl::hi32(T));
+}
+
+// state <- MDS * state + k (k: 12 constants, the next round's), with
+// MDS = circ(17,15,41,16,2,28,13,13,39,18,34,20) + diag(8,0,...,0).
+// Each output row is accumulated in 32-bit limbs, A = k_lo + sum c*lo(x),
+// B = k_hi + sum c*hi(x), one v_mad_u64_u32 per product (coefficients are
+// inline constants; row coefficient sums <= 264 keep A, B < 2^41).
+__device__ __forceinline__ void poseidon_mds(uint64_t s[12], const uint64_t *k)
+{
+    uint32_t lo[12], hi[12];
 #pragma unroll
     for (int i = 0; i < 12; i++) {
-        l0[i] = (uint32_t)s[i] & 0x3FFFFFu;
-        l1[i] = (uint32_t)(s[i] >> 22) & 0x3FFFFFu;
-        l2[i] = (uint32_t)(s[i] >> 44);
+        lo[i] = gl::lo32(s[i]);
+        hi[i] = gl::hi32(s[i]);
     }
 #pragma unroll
     for (int r = 0; r < 12; r++) {
-        uint32_t a0 = 0, a1 = 0, a2 = 0;
+        const uint64_t kr = k[r];
+        uint64_t A = gl::lo32(kr), B = gl::hi32(kr);
 #pragma unroll
         for (int i = 0; i < 12; i++) {
-            const int k = (i + r) % 12;
+            const int j = (i + r) % 12;
             const uint32_t c = (uint32_t)ZK_POSEIDON_MDS_CIRC[i] + (i == 0 ? (uint32_t)ZK_POSEIDON_MDS_DIAG[r] : 0u);
-            a0 = __umul24(l0[k], c) + a0;
-            a1 = __umul24(l1[k], c) + a1;
-            a2 = __umul24(l2[k], c) + a2;
+            A = (uint64_t)lo[j] * c + A;
+            B = (uint64_t)hi[j] * c + B;
         }
-        // value = a0 + a1*2^22 + a2*2^44  (< 2^75)
-        uint64_t lo = (uint64_t)a0 + ((uint64_t)a1 << 22);
-        uint64_t t = (uint64_t)a2 << 44;
-        uint64_t sum = lo + t;
-        uint32_t hi = (a2 >> 20) + (sum < t ? 1u : 0u);
-        s[r] = gl::reduce96(sum, hi);
+        s[r] = limb_combine(A, B);
     }
 }
 
 __device__ __forceinline__ void poseidon_permute(uint64_t s[12])
 {
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-#pragma unroll
-        for (int i = 0; i < 12; i++) s[i] = gl::pow7(gl::add(s[i], ZK_POSEIDON_RC[12 * r + i]));
-        poseidon_mds(s);
-    }
-#pragma unroll
-    for (int r = 4; r < 26; r++) {
-#pragma unroll
-        for (int i = 0; i < 12; i++) s[i] = gl::add(s[i], ZK_POSEIDON_RC[12 * r + i]);
+    for (int i = 0; i < 12; i++) s[i] = gl::add(s[i], c_poseidon_rc[i]);
+    // rounds 0-3 and 26-29 are full (S-box on all 12), 4-25 partial (s0 only)
+#pragma unroll 1
+    for (int r = 0; r < 30; r++) {
         s[0] = gl::pow7(s[0]);
-        poseidon_mds(s);
-    }
+        if (r < 4 || r >= 26) {
 #pragma unroll
-    for (int r = 26; r < 30; r++) {
-#pragma unroll
-        for (int i = 0; i < 12; i++) s[i] = gl::pow7(gl::add(s[i], ZK_POSEIDON_RC[12 * r + i]));
-        poseidon_mds(s);
+            for (int i = 1; i < 12; i++) s[i] = gl::pow7(s[i]);
+        }
+        poseidon_mds(s, c_poseidon_rc + 12 * (r + 1));
     }
 }
 
 // --------------------------------------------------------------- Poseidon2
 
-// One 4-block times M4 = [[5,7,1,3],[4,6,1,1],[1,3,5,7],[1,1,4,6]].
-__device__ __forceinline__ void p2_m4(uint64_t &x0, uint64_t &x1, uint64_t &x2, uint64_t &x3)
-{
-    uint64_t t0 = gl::add(x0, x1);
-    uint64_t t1 = gl::add(x2, x3);
-    uint64_t t2 = gl::add(gl::add(x1, x1), t1);
-    uint64_t t3 = gl::add(gl::add(x3, x3), t0);
-    uint64_t t1x2 = gl::add(t1, t1);
-    uint64_t t0x2 = gl::add(t0, t0);
-    uint64_t t4 = gl::add(gl::add(t1x2, t1x2), t3);
-    uint64_t t5 = gl::add(gl::add(t0x2, t0x2), t2);
-    x0 = gl::add(t3, t5);
-    x1 = t5;
-    x2 = gl::add(t2, t4);
-    x3 = t4;
-}
-
+// External layer M_E = [[2M4, M4, M4], [M4, 2M4, M4], [M4, M4, 2M4]] with
+// M4 = [[5,7,1,3],[4,6,1,1],[1,3,5,7],[1,1,4,6]] (poseidon2.cu:65-145: M4 on
+// each 4-block, then each position plus the sum of that position over the
+// blocks).  out_b = M4 * (s_b + u), u[c] = s[c] + s[4+c] + s[8+c]; the shared
+// M4 * u seeds every block's limb accumulators.
 __device__ __forceinline__ void p2_external(uint64_t s[12])
 {
+    constexpr uint32_t M4[4][4] = {{5, 7, 1, 3}, {4, 6, 1, 1}, {1, 3, 5, 7}, {1, 1, 4, 6}};
+    uint32_t lo[12], hi[12];
 #pragma unroll
-    for (int b = 0; b < 12; b += 4) p2_m4(s[b], s[b + 1], s[b + 2], s[b + 3]);
+    for (int i = 0; i < 12; i++) {
+        lo[i] = gl::lo32(s[i]);
+        hi[i] = gl::hi32(s[i]);
+    }
+    uint64_t VA[4], VB[4];
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-        uint64_t sum = gl::add(gl::add(s[c], s[4 + c]), s[8 + c]);
-        s[c] = gl::add(s[c], sum);
-        s[4 + c] = gl::add(s[4 + c], sum);
-        s[8 + c] = gl::add(s[8 + c], sum);
+    for (int r = 0; r < 4; r++) {
+        uint64_t A = 0, B = 0;
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+#pragma unroll
+            for (int b = 0; b < 12; b += 4) {
+                A = (uint64_t)lo[b + c] * M4[r][c] + A;
+                B = (uint64_t)hi[b + c] * M4[r][c] + B;
+            }
+        VA[r] = A;
+        VB[r] = B;
+    }
+#pragma unroll
+    for (int b = 0; b < 12; b += 4)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            uint64_t A = VA[r], B = VB[r];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                A = (uint64_t)lo[b + c] * M4[r][c] + A;
+                B = (uint64_t)hi[b + c] * M4[r][c] + B;
+            }
+            s[b + r] = limb_combine(A, B);
+        }
+}
+
+// Canonical sum of the 12 state elements: 32-bit carry chain into three
+// words, one reduction.
+__device__ __forceinline__ uint64_t sum12(const uint64_t s[12])
+{
+    uint32_t lo = 0, hi = 0, top = 0;
+#pragma unroll
+    for (int i = 0; i < 12; i++) {
+        uint32_t c1, c2, c3;
+        lo = __builtin_addc(lo, gl::lo32(s[i]), 0u, &c1);
+        hi = __builtin_addc(hi, gl::hi32(s[i]), c1, &c2);
+        top = __builtin_addc(top, 0u, c2, &c3);
+    }
+    return gl::reduce96(gl::pack(lo, hi), top);
+}
+
+// Internal layer: s_i <- s_i * d_i + sum(s) (d = MATRIX_DIAG_12 = diag - 1),
+// each as one 128-bit value reduced once.
+__device__ __forceinline__ void p2_internal(uint64_t s[12])
+{
+    const uint64_t sum = sum12(s);
+#pragma unroll
+    for (int i = 0; i < 12; i++) {
+        uint64_t lo, hi;
+        gl::mul128(s[i], ZK_POSEIDON2_DIAG[i], lo, hi);
+        uint32_t c1, c2, c3, c4;
+        const uint32_t l0 = __builtin_addc(gl::lo32(lo), gl::lo32(sum), 0u, &c1);
+        const uint32_t l1 = __builtin_addc(gl::hi32(lo), gl::hi32(sum), c1, &c2);
+        const uint32_t h0 = __builtin_addc(gl::lo32(hi), 0u, c2, &c3);
+        const uint32_t h1 = __builtin_addc(gl::hi32(hi), 0u, c3, &c4);
+        s[i] = gl::reduce128(gl::pack(l0, l1), gl::pack(h0, h1));
     }
 }
 
 __device__ __forceinline__ void poseidon2_permute(uint64_t s[12])
 {
     p2_external(s);
+    // rounds 0-3 and 26-29 external (all 12 S-boxes), 4-25 internal (s0 only)
+#pragma unroll 1
+    for (int r = 0; r < 30; r++) {
+        const uint64_t *rc = c_poseidon2_rc + 12 * r;
+        s[0] = gl::pow7(gl::add(s[0], rc[0]));
+        if (r < 4 || r >= 26) {
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-#pragma unroll
-        for (int i = 0; i < 12; i++) s[i] = gl::pow7(gl::add(s[i], ZK_POSEIDON2_RC[12 * r + i]));
-        p2_external(s);
-    }
-#pragma unroll
-    for (int r = 4; r < 26; r++) {
-        s[0] = gl::pow7(gl::add(s[0], ZK_POSEIDON2_RC[12 * r]));
-        uint64_t sum = s[0];
-#pragma unroll
-        for (int i = 1; i < 12; i++) sum = gl::add(sum, s[i]);
-#pragma unroll
-        for (int i = 0; i < 12; i++) {
-            uint64_t lo, hi;
-            gl::mul128(s[i], ZK_POSEIDON2_DIAG[i], lo, hi);
-            // s_i * d_i + sum as one 128-bit value, reduced once
-            uint64_t l2 = lo + sum;
-            hi += (l2 < lo) ? 1 : 0;
-            s[i] = gl::reduce128(l2, hi);
+            for (int i = 1; i < 12; i++) s[i] = gl::pow7(gl::add(s[i], rc[i]));
+            p2_external(s);
+        } else {
+            p2_internal(s);
         }
-    }
-#pragma unroll
-    for (int r = 26; r < 30; r++) {
-#pragma unroll
-        for (int i = 0; i < 12; i++) s[i] = gl::pow7(gl::add(s[i], ZK_POSEIDON2_RC[12 * r + i]));
-        p2_external(s);
     }
 }
 
@@ -178,8 +210,6 @@ __device__ __forceinline__ void hash_one(uint32_t n, Load load, uint64_t out[4])
     uint64_t s[12];
 #pragma unroll
     for (int i = 0; i < 12; i++) s[i] = 0;
-    // one permutation call site keeps the unrolled permutation in the
-    // instruction cache once per kernel
     for (uint32_t idx = 0; idx < n; idx += 8) {
 #pragma unroll
         for (int i = 0; i < 8; i++)

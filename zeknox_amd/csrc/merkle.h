@@ -1,6 +1,9 @@
 // Host-side Merkle forest driver (device pointers, enqueued on the device's
 // stream; the caller synchronises).
 #pragma once
+#include <vector>
+
+#include "merkle_types.h"
 #include "runtime.h"
 
 namespace zkx {
@@ -9,6 +12,7 @@ struct MerkleSpec {
     uint64_t *digests = nullptr;       // num_digests x 4
     uint64_t *caps = nullptr;          // num_caps x 4
     const uint64_t *leaves = nullptr;  // element k of leaf t at t*leaf_stride + k*col_stride
+    uint64_t *leaves_out = nullptr;    // optional row-major copy of the leaves
     uint64_t num_digests = 0, num_caps = 0, num_leaves = 0;
     uint64_t leaf_size = 0, cap_height = 0, hash_type = 0;
     uint64_t leaf_stride = 0, col_stride = 1;
@@ -16,6 +20,12 @@ struct MerkleSpec {
 
 // Throws Error(1, ...) when the sizes are not a linear Merkle forest.
 void check_merkle(const MerkleSpec &s);
+// Pass descriptors of one tree, in launch order (tile_begin left 0).
+void plan_merkle(const MerkleSpec &s, std::vector<MerkleArgs> &passes);
+// One launch over `n` job descriptors in device memory (n > 0), or over the
+// single descriptor `one` (n == 0); `blocks` work-groups in total.
+void launch_merkle_pass(hipStream_t stream, int hash_type, bool from_leaves, const MerkleArgs *dev_list,
+                        uint32_t n, const MerkleArgs &one, uint64_t blocks);
 void merkle_build(Device &d, const MerkleSpec &s);
 
 }  // namespace zkx

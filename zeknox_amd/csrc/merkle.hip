@@ -1,5 +1,6 @@
 // Host side of the Merkle forest: argument checks mirroring merkle.cpp:19-124
-// and merkle.cu:165-244, and the pass schedule of merkle_kernels.h.
+// and merkle.cu:165-244, the pass schedule of merkle_kernels.h, and launches
+// of pass lists (one tree, or many trees of a commit batch).
 #include "merkle.h"
 
 #include <algorithm>
@@ -13,50 +14,6 @@ namespace {
 
 bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
 uint32_t log2u(uint64_t x) { return 63 - __builtin_clzll(x); }
-
-template <int HT>
-void launch_tree(Device &d, const MerkleSpec &s)
-{
-    MerkleArgs a{};
-    a.leaves = s.leaves;
-    a.dig = s.digests;
-    a.cap = s.caps;
-    a.leaf_size = (uint32_t)s.leaf_size;
-    a.leaf_stride = s.leaf_stride;
-    a.col_stride = s.col_stride;
-    if (s.num_caps == s.num_leaves) {
-        a.count = s.num_leaves;
-        const uint64_t blocks = (a.count + kMerkleThreads - 1) / kMerkleThreads;
-        ProfScope ps("merkle_leaves", d.stream);
-        hipLaunchKernelGGL(k_leaf_direct<HT>, dim3((uint32_t)blocks), dim3(kMerkleThreads), 0, d.stream, a);
-        ZKX_HIP(hipGetLastError());
-        return;
-    }
-    const uint32_t h = log2u(s.num_caps);
-    uint32_t lg_w = log2u(s.num_leaves) - h;  // per-subtree width of the input level
-    a.sdl = s.num_digests >> h;
-    bool first = true;
-    while (true) {
-        const uint64_t count = s.num_caps << lg_w;
-        const uint32_t tile_log = std::min<uint32_t>(8, log2u(count));
-        a.count = count;
-        a.lg_w = lg_w;
-        a.tile_log = tile_log;
-        const uint64_t blocks = count >> tile_log;
-        if (blocks > 0x7FFFFFFFull) invalid("tree too large for one launch");
-        ProfScope ps(first ? "merkle_leaves" : "merkle_levels", d.stream);
-        if (first)
-            hipLaunchKernelGGL((k_merkle_tile<HT, true>), dim3((uint32_t)blocks), dim3(kMerkleThreads), 0,
-                               d.stream, a);
-        else
-            hipLaunchKernelGGL((k_merkle_tile<HT, false>), dim3((uint32_t)blocks), dim3(kMerkleThreads), 0,
-                               d.stream, a);
-        ZKX_HIP(hipGetLastError());
-        first = false;
-        if (lg_w <= tile_log) break;  // this pass reached the caps
-        lg_w -= tile_log;
-    }
-}
 
 }  // namespace
 
@@ -75,11 +32,64 @@ void check_merkle(const MerkleSpec &s)
     if (!s.digests) invalid("null digest buffer");
 }
 
+void plan_merkle(const MerkleSpec &s, std::vector<MerkleArgs> &passes)
+{
+    MerkleArgs a{};
+    a.leaves = s.leaves;
+    a.leaves_out = s.leaves_out;
+    a.dig = s.digests;
+    a.cap = s.caps;
+    a.leaf_size = (uint32_t)s.leaf_size;
+    a.leaf_stride = s.leaf_stride;
+    a.col_stride = s.col_stride;
+    if (s.num_caps == s.num_leaves) {
+        a.count = s.num_leaves;
+        a.lg_w = 0;
+        a.tile_log = std::min<uint32_t>(8, log2u(a.count));
+        passes.push_back(a);
+        return;
+    }
+    const uint32_t h = log2u(s.num_caps);
+    uint32_t lg_w = log2u(s.num_leaves) - h;  // per-subtree width of the input level
+    a.sdl = s.num_digests >> h;
+    while (true) {
+        a.count = s.num_caps << lg_w;
+        a.lg_w = lg_w;
+        a.tile_log = std::min<uint32_t>(8, log2u(a.count));
+        passes.push_back(a);
+        if (lg_w <= a.tile_log) break;  // this pass reaches the caps
+        lg_w -= a.tile_log;
+    }
+}
+
+void launch_merkle_pass(hipStream_t stream, int hash_type, bool from_leaves, const MerkleArgs *dev_list,
+                        uint32_t n, const MerkleArgs &one, uint64_t blocks)
+{
+    if (blocks == 0) return;
+    if (blocks > 0x7FFFFFFFull) invalid("tree too large for one launch");
+    ProfScope ps(from_leaves ? "merkle_leaves" : "merkle_levels", stream);
+    const dim3 g((uint32_t)blocks), b(kMerkleThreads);
+    if (hash_type == kPoseidon2) {
+        if (from_leaves) hipLaunchKernelGGL((k_merkle_tile<kPoseidon2, true>), g, b, 0, stream, dev_list, n, one);
+        else hipLaunchKernelGGL((k_merkle_tile<kPoseidon2, false>), g, b, 0, stream, dev_list, n, one);
+    } else {
+        if (from_leaves) hipLaunchKernelGGL((k_merkle_tile<kPoseidon, true>), g, b, 0, stream, dev_list, n, one);
+        else hipLaunchKernelGGL((k_merkle_tile<kPoseidon, false>), g, b, 0, stream, dev_list, n, one);
+    }
+    ZKX_HIP(hipGetLastError());
+}
+
 void merkle_build(Device &d, const MerkleSpec &s)
 {
     check_merkle(s);
-    if (s.hash_type == 3) launch_tree<kPoseidon2>(d, s);
-    else launch_tree<kPoseidon>(d, s);
+    std::vector<MerkleArgs> passes;
+    plan_merkle(s, passes);
+    bool first = true;
+    for (MerkleArgs &p : passes) {
+        p.tile_begin = 0;
+        launch_merkle_pass(d.stream, (int)s.hash_type, first, nullptr, 0, p, p.count >> p.tile_log);
+        first = false;
+    }
 }
 
 }  // namespace zkx

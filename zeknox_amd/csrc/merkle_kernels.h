@@ -15,25 +15,21 @@
 // stops below the top leaves its root for the next pass, which starts from
 // that level; passes repeat until the caps are written (one pass when a
 // subtree has <= 256 leaves, two up to 2^16, three up to 2^24).
+//
+// Leaf sources.  Element k of leaf t is read at t*leaf_stride + k*col_stride:
+// row-major leaves (the C ABI) use (leaf_size, 1); the commit engine hashes
+// the LDE's bit-reversed column-major output directly with (1, N), so lanes
+// of a wave read consecutive words of each column, and optionally writes the
+// leaf rows (the transpose) on the way.
+//
+// A launch covers a list of independent jobs (trees or tree passes);
+// work-group b runs tile b - tile_begin of the job with the largest
+// tile_begin <= b.
 #pragma once
 #include "hash.h"
+#include "merkle_types.h"
 
 namespace zkx {
-
-constexpr int kMerkleThreads = 256;
-
-struct MerkleArgs {
-    const uint64_t *leaves;  // pass 0: leaf source
-    uint64_t *dig;
-    uint64_t *cap;
-    uint64_t count;          // nodes (or leaves) at the input level, all subtrees
-    uint64_t sdl;            // digests per subtree
-    uint32_t lg_w;           // log2 width of the input level per subtree
-    uint32_t tile_log;       // log2 nodes per work-group tile
-    uint32_t leaf_size;      // pass 0
-    uint64_t leaf_stride;    // u64 between consecutive leaves (row-major: leaf_size)
-    uint64_t col_stride;     // u64 between consecutive elements of a leaf (row-major: 1)
-};
 
 // Digest slot of node `idx` of the level of width 2^lg_w in subtree `sub`.
 __device__ __forceinline__ uint64_t *node_ptr(const MerkleArgs &a, uint64_t sub, uint32_t lg_w, uint64_t idx)
@@ -41,14 +37,27 @@ __device__ __forceinline__ uint64_t *node_ptr(const MerkleArgs &a, uint64_t sub,
     return a.dig + 4 * (sub * a.sdl + ((1ull << lg_w) - 2) + idx);
 }
 
+__device__ __forceinline__ const MerkleArgs *find_job(const MerkleArgs *jobs, uint32_t njobs)
+{
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (jobs[mid].tile_begin <= blockIdx.x) lo = mid;
+        else hi = mid;
+    }
+    return jobs + lo;
+}
+
 template <int HT, bool FROM_LEAVES>
-__global__ void __launch_bounds__(kMerkleThreads) k_merkle_tile(MerkleArgs a)
+__global__ void __launch_bounds__(kMerkleThreads) k_merkle_tile(const MerkleArgs *__restrict__ jobs, uint32_t njobs,
+                                                              MerkleArgs one)
 {
     __shared__ uint64_t lds[kMerkleThreads][4];
+    const MerkleArgs a = njobs ? *find_job(jobs, njobs) : one;
     const uint32_t t = threadIdx.x;
-    uint32_t cur = 1u << a.tile_log;     // nodes of this tile at the current level
-    uint32_t lg_w = a.lg_w;              // log2 per-subtree width of the current level
-    uint64_t first = (uint64_t)blockIdx.x << a.tile_log;  // first node of the tile
+    uint32_t cur = 1u << a.tile_log;  // nodes of this tile at the current level
+    uint32_t lg_w = a.lg_w;           // log2 per-subtree width of the current level
+    uint64_t first = ((uint64_t)blockIdx.x - a.tile_begin) << a.tile_log;  // first node of the tile
 
     uint64_t h[4];
     if (t < cur) {
@@ -57,12 +66,17 @@ __global__ void __launch_bounds__(kMerkleThreads) k_merkle_tile(MerkleArgs a)
         if constexpr (FROM_LEAVES) {
             const uint64_t *row = a.leaves + g * a.leaf_stride;
             const uint64_t cs = a.col_stride;
-            hash_one<HT>(a.leaf_size, [&](uint32_t k) { return row[(uint64_t)k * cs]; }, h);
-            if (lg_w >= 1) {
-                uint64_t *dst = node_ptr(a, sub, lg_w, idx);
+            uint64_t *out = a.leaves_out ? a.leaves_out + g * a.leaf_size : nullptr;
+            hash_one<HT>(a.leaf_size,
+                         [&](uint32_t k) {
+                             const uint64_t v = row[(uint64_t)k * cs];
+                             if (out) out[k] = v;
+                             return v;
+                         },
+                         h);
+            uint64_t *dst = lg_w ? node_ptr(a, sub, lg_w, idx) : a.cap + 4 * g;
 #pragma unroll
-                for (int i = 0; i < 4; i++) dst[i] = h[i];
-            }
+            for (int i = 0; i < 4; i++) dst[i] = h[i];
         } else {
             const uint64_t *src = node_ptr(a, sub, lg_w, idx);
 #pragma unroll
@@ -71,6 +85,7 @@ __global__ void __launch_bounds__(kMerkleThreads) k_merkle_tile(MerkleArgs a)
 #pragma unroll
         for (int i = 0; i < 4; i++) lds[t][i] = h[i];
     }
+    if (lg_w == 0) return;  // caps == leaves: done
     __syncthreads();
 
     while (true) {
@@ -106,20 +121,6 @@ __global__ void __launch_bounds__(kMerkleThreads) k_merkle_tile(MerkleArgs a)
         cur = half;
         __syncthreads();
     }
-}
-
-// caps == leaves: cap[i] = hash_one(leaf i)  (merkle.cu:23-38 direct case).
-template <int HT>
-__global__ void __launch_bounds__(kMerkleThreads) k_leaf_direct(MerkleArgs a)
-{
-    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= a.count) return;
-    const uint64_t *row = a.leaves + g * a.leaf_stride;
-    const uint64_t cs = a.col_stride;
-    uint64_t h[4];
-    hash_one<HT>(a.leaf_size, [&](uint32_t k) { return row[(uint64_t)k * cs]; }, h);
-#pragma unroll
-    for (int i = 0; i < 4; i++) a.cap[4 * g + i] = h[i];
 }
 
 }  // namespace zkx

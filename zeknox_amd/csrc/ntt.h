@@ -1,6 +1,9 @@
 // Host-side NTT / LDE / transpose drivers (device pointers, enqueued on the
 // device's stream; the caller synchronises).
 #pragma once
+#include <vector>
+
+#include "ntt_types.h"
 #include "runtime.h"
 
 namespace zkx {
@@ -9,6 +12,9 @@ namespace zkx {
 // each output column in bit-reversed order (DIF order, no reversal pass).
 void lde_bitrev(Device &d, uint64_t *out, const uint64_t *in, uint32_t lg_n, uint32_t rate, uint32_t cols,
                 bool inverse, bool with_coset);
+// Resident LDE factor table F[b*n + i] = (g * w_N^{bitrev_r(b)})^i (g = 1
+// without coset; w^-1 for the inverse), built on first use.
+const uint64_t *lde_factor_table(Device &d, uint32_t lg_n, uint32_t rate, bool inverse, bool with_coset);
 // compute_batched_lde semantics: natural-order output.
 void lde(Device &d, uint64_t *out, const uint64_t *in, uint32_t lg_n, uint32_t rate, uint32_t cols,
          bool inverse, bool with_coset);
@@ -19,6 +25,16 @@ void bitrev_columns(Device &d, uint64_t *buf, uint32_t lg, uint32_t cols, bool u
                     const uint64_t *coset);
 // out[t * cols + b] = in[b * 2^lg + (rev ? bitrev(t) : t)].
 void transpose_rev(Device &d, uint64_t *out, const uint64_t *in, uint32_t lg, uint32_t cols, bool rev);
+
+// Pass descriptors of the DIF over nblk blocks of 2^L in buf (in launch
+// order; each pass must finish before the next starts).  Returns the tile
+// count of every pass.
+uint64_t plan_dif(std::vector<NttPass> &out, uint64_t *buf, uint64_t nblk, uint32_t L, uint32_t lg_tw,
+                  const uint64_t *tw, bool inverse, uint32_t mode0, const uint64_t *src, uint32_t rate,
+                  bool with_coset, const uint64_t *coset);
+
+// One launch over n pass descriptors in device memory (sorted by tile_begin).
+void launch_ntt_list(hipStream_t stream, const NttPass *dev_list, uint32_t n, uint64_t blocks, const char *name);
 
 // table[i] = base^i, i < n.
 void fill_powers(uint64_t *dst, uint64_t n, uint64_t base, hipStream_t s);

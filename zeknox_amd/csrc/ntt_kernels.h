@@ -7,43 +7,24 @@
 //
 // Algorithm.  The forward transform is decimation-in-frequency (natural
 // order in, bit-reversed order out), run as a few passes over HBM.  Each pass
-// performs up to 12 consecutive radix-2 stages on tiles held in LDS: a tile is
-// 2^m elements at stride 2^s_lo (the butterfly partners of stages
-// s_lo..s_lo+m-1) times 2^c consecutive columns of that stride, so every
-// global load and store is a run of 2^c consecutive u64.
+// performs up to 12 consecutive radix-2 stages on a tile of 4096 elements in
+// LDS: 2^m elements at stride 2^s_lo (the butterfly partners of stages
+// s_lo..s_lo+m-1) times 2^c consecutive columns of that stride (m + c = 12),
+// so every global load and store is a run of 2^c consecutive u64.  Inside
+// the tile, each lane holds 16 elements in VGPRs and runs 4 stages per round
+// (radix 16) between LDS exchanges.
 //
 // LDE.  Zero-padding n = 2^lg_n coefficients to N = 2^(lg_n+r) and running the
 // first r DIF stages leaves, in block b of length n (b < 2^r, memory order),
-// the values a_i * (w_N^{bitrev_r(b)})^i: the first pass reads the n
-// coefficients directly and applies that factor (and the coset factor g^i),
-// so the zero half is never read, written or butterflied.
+// the values a_i * F[b*n + i] with F[b*n + i] = (g * w_N^{bitrev_r(b)})^i
+// (g = 1 without coset; w^-1 for the inverse): the first pass reads the n
+// coefficients directly and multiplies by the resident factor table F, so
+// the zero half is never read, written or butterflied.
 #pragma once
 #include "gl64.h"
+#include "ntt_types.h"
 
 namespace zkx {
-
-constexpr int kNttTileLog = 12;  // 4096 u64 = 32 KB of LDS per work-group
-constexpr int kNttThreads = 256;
-
-enum SrcMode : int {
-    kSrcInPlace = 0,  // tile source is the buffer itself
-    kSrcCoset = 1,    // buffer itself, times g^i (forward coset NTT)
-    kSrcLde = 2,      // coefficients of the un-extended columns (LDE first pass)
-};
-
-struct PassArgs {
-    uint64_t *buf;          // flat array of nblk blocks of 2^L
-    const uint64_t *src;    // kSrcLde: coefficients [cols][2^L]
-    const uint64_t *tw;     // w_T^k, k < 2^lg_tw (w_T = primitive 2^lg_tw-th root)
-    const uint64_t *coset;  // g^i table (kSrcCoset / kSrcLde with coset)
-    uint64_t total;         // nblk << L
-    uint32_t L;             // log2 block length
-    uint32_t lg_tw;         // log2 of twiddle table length (>= L + rate)
-    uint32_t rate;          // kSrcLde: log2 of blocks per column
-    uint32_t s_lo, m, c;    // stages s_lo .. s_lo+m-1; 2^c columns per tile
-    uint32_t inverse;       // use w^-1
-    uint32_t with_coset;    // kSrcLde forward: multiply by g^i
-};
 
 __device__ __forceinline__ uint64_t tw_at(const uint64_t *tw, uint32_t lg_tw, uint64_t k, bool inverse)
 {
@@ -56,87 +37,163 @@ __device__ __forceinline__ uint32_t brev_bits(uint32_t x, uint32_t bits)
     return bits ? (__builtin_bitreverse32(x) >> (32 - bits)) : 0u;
 }
 
-// Memory index (flat) of tile element (j, q) for tile id `tile`.
-__device__ __forceinline__ uint64_t pass_index(const PassArgs &a, uint64_t tile, uint32_t j, uint32_t q)
+// LDS slot of tile position u (one pad word per 32 breaks power-of-two strides)
+__device__ __forceinline__ uint32_t lds_slot(uint32_t u) { return u + (u >> 5); }
+
+// Tile positions are numbered u in memory order: u = j*2^c + q when the
+// pass is strided (s_lo > 0: column q of the stride, butterfly index j),
+// u = q*2^m + j when it is contiguous (s_lo == 0: group q, index j).
+__device__ __forceinline__ uint64_t tile_index(const NttPass &a, uint64_t tile, uint32_t u)
 {
     if (a.s_lo > 0) {
-        // groups of 2^(s_lo+m) elements; in a group, (j, lo) -> j*2^s_lo + lo;
-        // 2^(s_lo-c) column chunks per group
-        const uint32_t chunk_bits = a.s_lo - a.c;
+        const uint32_t q = u & ((1u << a.c) - 1), j = u >> a.c;
+        const uint32_t chunk_bits = a.s_lo - a.c;  // column chunks per group
         const uint64_t g = tile >> chunk_bits, ch = tile & ((1ull << chunk_bits) - 1);
         return (g << (a.s_lo + a.m)) + ((uint64_t)j << a.s_lo) + (ch << a.c) + q;
     }
-    // s_lo == 0: 2^c consecutive contiguous groups of 2^m
-    return (((tile << a.c) + q) << a.m) + j;
+    return (tile << kNttTileLog) + u;
 }
 
-// In-block position modulo 2^s needed for the twiddle exponent of the upper
-// butterfly element: ((j mod 2^d) << s_lo) | lo.
-template <int MODE>
-__global__ void __launch_bounds__(kNttThreads) k_dif_pass(PassArgs a)
+// One register round: stages on j-bits [b0, b0+K) (u-bits [RB, RB+K)).
+// Lane t holds the 2^K elements that differ in those bits, repeated over
+// 2^(4-K) groups taken from the extra element bits; the other u-bits come
+// from the lane id.  The twiddle of stage d for element e depends only on
+// (j mod 2^d) << s_lo | lo.
+template <int K>
+__device__ __forceinline__ void ntt_round(uint64_t *lds, const NttPass &a, uint32_t t, uint32_t b0, uint32_t jpos,
+                                          uint64_t lo_base)
 {
-    __shared__ uint64_t lds[1 << kNttTileLog];
-    const uint32_t tsz_log = a.m + a.c;
-    const uint32_t tsz = 1u << tsz_log;
-    const uint64_t tile = blockIdx.x;
-    const uint32_t cmask = (1u << a.c) - 1;
-    const uint32_t mmask = (1u << a.m) - 1;
+    constexpr int E = kNttPerThread;
+    const uint32_t RB = jpos + b0;
+    uint64_t x[E];
+    uint32_t uu[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const uint32_t rb = (uint32_t)e & ((1u << K) - 1);
+        const uint32_t other = t | (((uint32_t)e >> K) << 8);
+        const uint32_t u = ((other >> RB) << (RB + K)) | (rb << RB) | (other & ((1u << RB) - 1));
+        uu[e] = u;
+        x[e] = lds[lds_slot(u)];
+    }
+#pragma unroll
+    for (int dd = K - 1; dd >= 0; dd--) {
+        const uint32_t d = b0 + (uint32_t)dd;
+        const uint32_t shift = a.lg_tw - (a.s_lo + d) - 1;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            if ((e >> dd) & 1) continue;
+            const int f = e | (1 << dd);
+            const uint32_t u = uu[e];
+            const uint32_t j = a.s_lo > 0 ? (u >> a.c) : (u & ((1u << a.m) - 1));
+            const uint64_t lo_part = a.s_lo > 0 ? lo_base + (u & ((1u << a.c) - 1)) : 0;
+            const uint64_t ex = ((((uint64_t)(j & ((1u << d) - 1))) << a.s_lo) | lo_part) << shift;
+            const uint64_t y0 = x[e], y1 = x[f];
+            x[e] = gl::add(y0, y1);
+            x[f] = gl::mul(gl::sub(y0, y1), tw_at(a.tw, a.lg_tw, ex, a.inverse));
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < E; e++) lds[lds_slot(uu[e])] = x[e];
+}
 
-    // ---- load (consecutive threads -> consecutive memory)
-    for (uint32_t e = threadIdx.x; e < tsz; e += kNttThreads) {
-        uint32_t j, q;
-        if (a.s_lo > 0) { q = e & cmask; j = e >> a.c; }
-        else { j = e & mmask; q = e >> a.m; }
-        const uint64_t idx = pass_index(a, tile, j, q);
+// Full-width round (K = 4, no extra element bits): the lane's other u-bits
+// are its id, so the 15 distinct twiddles of the round are loaded once.
+__device__ __forceinline__ void ntt_round4(uint64_t *lds, const NttPass &a, uint32_t t, uint32_t b0, uint32_t jpos,
+                                           uint64_t lo_base)
+{
+    constexpr int E = kNttPerThread;
+    const uint32_t RB = jpos + b0;
+    const uint32_t ubase = ((t >> RB) << (RB + 4)) | (t & ((1u << RB) - 1));
+    uint64_t x[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) x[e] = lds[lds_slot(ubase | ((uint32_t)e << RB))];
+    // lane constants: j-bits below b0 and the column's low in-block bits
+    const uint32_t j0 = (a.s_lo > 0 ? (ubase >> a.c) : (ubase & ((1u << a.m) - 1))) & ((1u << b0) - 1);
+    const uint64_t lo_part = a.s_lo > 0 ? lo_base + (ubase & ((1u << a.c) - 1)) : 0;
+    const uint64_t base = ((uint64_t)j0 << a.s_lo) | lo_part;
+#pragma unroll
+    for (int dd = 3; dd >= 0; dd--) {
+        const uint32_t d = b0 + (uint32_t)dd;
+        const uint32_t shift = a.lg_tw - (a.s_lo + d) - 1;
+        uint64_t w[1 << dd];
+#pragma unroll
+        for (int v = 0; v < (1 << dd); v++)
+            w[v] = tw_at(a.tw, a.lg_tw, (base + ((uint64_t)v << (b0 + a.s_lo))) << shift, a.inverse);
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            if ((e >> dd) & 1) continue;
+            const int f = e | (1 << dd);
+            const uint64_t y0 = x[e], y1 = x[f];
+            x[e] = gl::add(y0, y1);
+            x[f] = gl::mul(gl::sub(y0, y1), w[e & ((1 << dd) - 1)]);
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < E; e++) lds[lds_slot(ubase | ((uint32_t)e << RB))] = x[e];
+}
+
+// `passes` (npasses > 0, device memory) lists independent passes sorted by
+// tile_begin; with npasses == 0 the launch runs the single pass `one`.
+__global__ void __launch_bounds__(kNttThreads) k_ntt_tiles(const NttPass *__restrict__ passes, uint32_t npasses,
+                                                         NttPass one)
+{
+    __shared__ uint64_t lds[(1 << kNttTileLog) + (1 << kNttTileLog) / 32];
+
+    // locate this work-group's pass (uniform)
+    uint32_t lo = 0, hi = npasses;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (passes[mid].tile_begin <= blockIdx.x) lo = mid;
+        else hi = mid;
+    }
+    const NttPass a = npasses ? passes[lo] : one;
+    const uint64_t tile = blockIdx.x - a.tile_begin;
+    const uint32_t t = threadIdx.x;
+
+    // ---- load, coalesced: lane t takes positions t + 256 i
+#pragma unroll
+    for (int i = 0; i < kNttPerThread; i++) {
+        const uint32_t u = t + kNttThreads * i;
+        const uint64_t idx = tile_index(a, tile, u);
         uint64_t v = 0;
         if (idx < a.total) {
-            if constexpr (MODE == kSrcLde) {
+            if (a.mode == kSrcLde) {
+                // first `rate` DIF stages of the zero-padded column: block b of
+                // column col holds a_i * F[b*n + i], F the resident factor table
                 const uint64_t n = 1ull << a.L;
-                const uint64_t blk = idx >> a.L, i = idx & (n - 1);
-                const uint64_t col = blk >> a.rate;
-                const uint32_t b = (uint32_t)(blk & ((1u << a.rate) - 1));
-                v = a.src[(col << a.L) + i];
-                // factor (w_N^{bitrev_r(b)})^i, plus g^i for a forward coset LDE
-                const uint64_t e_tw = ((uint64_t)brev_bits(b, a.rate) * i) << (a.lg_tw - a.L - a.rate);
-                if (e_tw) v = gl::mul(v, tw_at(a.tw, a.lg_tw, e_tw, a.inverse));
-                if (a.with_coset && !a.inverse) v = gl::mul(v, a.coset[i]);
+                const uint64_t col = idx >> (a.L + a.rate), pos = idx & (n - 1);
+                v = gl::mul(a.src[(col << a.L) + pos], a.coset[idx & ((n << a.rate) - 1)]);
             } else {
                 v = a.buf[idx];
-                if constexpr (MODE == kSrcCoset) v = gl::mul(v, a.coset[idx & ((1ull << a.L) - 1)]);
+                if (a.mode == kSrcCoset) v = gl::mul(v, a.coset[idx & ((1ull << a.L) - 1)]);
             }
         }
-        lds[(j << a.c) + q] = v;
+        lds[lds_slot(u)] = v;
     }
     __syncthreads();
 
-    // ---- stages s = s_lo+m-1 .. s_lo
-    const uint32_t nbf = tsz >> 1;
-    // low in-block bits shared by the tile's columns (strided passes only)
+    // low in-block bits shared by the tile's columns (strided passes)
     const uint64_t lo_base = a.s_lo > 0 ? ((tile & ((1ull << (a.s_lo - a.c)) - 1)) << a.c) : 0;
-    for (int d = (int)a.m - 1; d >= 0; d--) {
-        const uint32_t s = a.s_lo + (uint32_t)d;
-        const uint32_t tw_shift = a.lg_tw - s - 1;
-        for (uint32_t u = threadIdx.x; u < nbf; u += kNttThreads) {
-            const uint32_t q = u & cmask, jj = u >> a.c;
-            const uint32_t j = ((jj >> d) << (d + 1)) | (jj & ((1u << d) - 1));
-            const uint64_t lo_part = a.s_lo > 0 ? lo_base + q : 0;
-            const uint64_t ex = (((uint64_t)(j & ((1u << d) - 1)) << a.s_lo) | lo_part) << tw_shift;
-            const uint32_t i0 = (j << a.c) + q, i1 = ((j + (1u << d)) << a.c) + q;
-            const uint64_t x = lds[i0], y = lds[i1];
-            lds[i0] = gl::add(x, y);
-            const uint64_t diff = gl::sub(x, y);
-            lds[i1] = ex ? gl::mul(diff, tw_at(a.tw, a.lg_tw, ex, a.inverse)) : diff;
-        }
+    const uint32_t jpos = a.s_lo > 0 ? a.c : 0;  // u-bit of j-bit 0
+
+    // ---- register rounds, top j-bits first: 4 stages each, then the rest
+    uint32_t rem = a.m;
+    while (rem >= 4) {
+        rem -= 4;
+        ntt_round4(lds, a, t, rem, jpos, lo_base);
         __syncthreads();
     }
+    if (rem == 3) ntt_round<3>(lds, a, t, 0, jpos, lo_base);
+    else if (rem == 2) ntt_round<2>(lds, a, t, 0, jpos, lo_base);
+    else if (rem == 1) ntt_round<1>(lds, a, t, 0, jpos, lo_base);
+    if (rem) __syncthreads();
 
-    // ---- store
-    for (uint32_t e = threadIdx.x; e < tsz; e += kNttThreads) {
-        uint32_t j, q;
-        if (a.s_lo > 0) { q = e & cmask; j = e >> a.c; }
-        else { j = e & mmask; q = e >> a.m; }
-        const uint64_t idx = pass_index(a, tile, j, q);
-        if (idx < a.total) a.buf[idx] = lds[(j << a.c) + q];
+    // ---- store, coalesced
+#pragma unroll
+    for (int i = 0; i < kNttPerThread; i++) {
+        const uint32_t u = t + kNttThreads * i;
+        const uint64_t idx = tile_index(a, tile, u);
+        if (idx < a.total) a.buf[idx] = lds[lds_slot(u)];
     }
 }
 
@@ -259,6 +316,30 @@ __global__ void __launch_bounds__(kNttThreads) k_salt(uint64_t *out, uint64_t co
         z ^= z >> 31;
     } while (z >= gl::P);
     out[g] = z;
+}
+
+// LDE factor table: F[b*n + i] = c_b^i, c_b = bases[b], b < 2^rate, i < n.
+struct FactorBases {
+    uint64_t c[1 << kMaxRateBits];
+};
+
+__global__ void __launch_bounds__(kNttThreads) k_lde_factors(uint64_t *F, uint32_t lg_n, uint32_t rate,
+                                                           FactorBases bases)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t n = 1ull << lg_n;
+    const uint64_t chunk = n < 64 ? n : 64;  // runs never cross a block
+    const uint64_t start = t * chunk;
+    if (start >= (n << rate)) return;
+    const uint32_t b = (uint32_t)(start >> lg_n);
+    const uint64_t i0 = start & (n - 1);
+    const uint64_t c = bases.c[b];
+    uint64_t v = gl::pow(c, i0);
+    const uint64_t end = i0 + chunk;
+    for (uint64_t i = i0; i < end; i++) {
+        F[((uint64_t)b << lg_n) + i] = v;
+        v = gl::mul(v, c);
+    }
 }
 
 // table[i] = base^i for i < n, 64 consecutive powers per thread.

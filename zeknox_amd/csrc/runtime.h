@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
+#include <tuple>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -66,6 +68,8 @@ struct Device {
     uint64_t *coset = nullptr;     // coset[i] = g^i, i < 2^coset_lg
     uint32_t coset_lg = 0;
     uint64_t coset_gen = 0;
+    // LDE factor tables keyed by (lg_n, rate, inverse, generator)
+    std::map<std::tuple<uint32_t, uint32_t, bool, uint64_t>, uint64_t *> lde_factors;
 
     const uint64_t *twiddle_table(uint32_t lg);  // builds on first use
     void build_coset(uint32_t lg, uint64_t gen);
