@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--jobs", type=int, default=4, help="config-2 commits per batched step")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="served-stream duration (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=40.0, help="served-stream arrivals per second")
     ap.add_argument("--cpu-rows", type=int, default=1 << 13,
                     help="leaf rows of the bounded CPU sample (of 2^18)")
     return ap.parse_args()
@@ -58,6 +60,67 @@ def perms_per_commit(rows, cols, caps):
     # internal node and cap: rows - caps
     per_leaf = (cols + 7) // 8 if cols > 4 else 0
     return rows * per_leaf + (rows - caps)
+
+
+def served_stream(zk, np, seconds, rate_hz, budget, streams, seed=5):
+    """Poisson arrivals of heterogeneous commit requests into the commit server
+    for `seconds`: log2 degree lognormal clipped to [12, 20], columns uniform
+    in [2, 400], rate_bits in {1, 2, 3}, Poseidon / Poseidon2 50/50, cap
+    height 4, leaf rows kept.  Returns throughput and latency percentiles
+    (latency = completion - submission, steady clock inside the server)."""
+    import threading
+    rng = np.random.default_rng(seed)
+    # one device pool of coefficients serves every request (its first cols*2^lg_n words)
+    pool = zk.DeviceBuffer(400 << 20)
+    pool.zero()
+    srv = zk.Server(budget, streams=streams, max_batch=64)
+    arrivals, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / rate_hz)
+        if t >= seconds:
+            break
+        lg_n = int(np.clip(np.rint(rng.lognormal(np.log(14.0), 0.15)), 12, 20))
+        cols = int(rng.integers(2, 401))
+        rate = int(rng.integers(1, 4))
+        ht = 0 if rng.random() < 0.5 else 3
+        arrivals.append((t, cols, lg_n, rate, ht))
+    ids = []
+    done = threading.Event()
+    lat, t_first, t_last = [], [None], [None]
+
+    def consumer():
+        i = 0
+        while i < len(arrivals):
+            while i >= len(ids):
+                time.sleep(0.0005)
+            r = srv.wait(ids[i])
+            lat.append(r.t_done - r.t_submit)
+            t_last[0] = r.t_done
+            srv.release(ids[i])
+            i += 1
+        done.set()
+
+    th = threading.Thread(target=consumer)
+    th.start()
+    t0 = time.perf_counter()
+    for (ta, cols, lg_n, rate, ht) in arrivals:
+        delay = t0 + ta - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(srv.submit(pool, cols, lg_n, rate, 4, ht))
+    done.wait()
+    th.join()
+    elapsed = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat_ms = np.array(lat) * 1e3
+    return {"workload": f"Poisson {rate_hz}/s for {seconds} s; log2 degree lognormal(ln 14, 0.15) clipped [12,20], "
+                        "cols U[2,400], rate_bits {1,2,3}, Poseidon/Poseidon2 50/50, cap_height 4",
+            "requests": len(arrivals), "commits_per_s": round(len(arrivals) / elapsed, 2),
+            "p50_ms": round(float(np.percentile(lat_ms, 50)), 2), "p99_ms": round(float(np.percentile(lat_ms, 99)), 2),
+            "mean_ms": round(float(lat_ms.mean()), 2), "batches": st["batches"], "max_batch": st["max_batch"],
+            "peak_hbm_gb": round(st["peak_bytes"] / 2**30, 2), "budget_gb": round(budget / 2**30, 1),
+            "streams": streams}
 
 
 def main():
@@ -189,6 +252,10 @@ def main():
                "sample": f"full LDE+transpose of 135x2^15 -> 2^18 ({lde_s:.2f} s) + Poseidon Merkle over "
                          f"{rows} of 2^18 leaf rows ({merkle_s:.2f} s), scaled to one commit"}
 
+    served = None
+    if args.serve_seconds > 0:
+        served = served_stream(zk, np, args.serve_seconds, args.serve_rate, 64 << 30, 3)
+
     value = args.jobs / batch_step_s
     out = {
         "metric": "LDE+Merkle commits/s batched vs plain at bit-exact; p50/p99 commit latency",
@@ -215,6 +282,7 @@ def main():
         "roofline": roofline,
         "ntt_roofline": ntt_roofline,
         "cpu_baseline": cpu,
+        "served": served,
     }
     print(json.dumps(out))
 

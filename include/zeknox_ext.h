@@ -54,6 +54,34 @@ RustError zkx_commit_batch(size_t device_id, const ZkxCommitJob *jobs, size_t nj
 /* Device working memory zkx_commit_batch needs for these jobs, in bytes. */
 size_t zkx_commit_scratch_bytes(const ZkxCommitJob *jobs, size_t njobs);
 
+/* ---- commit server (continuous batching under an HBM budget) ------------
+ * Requests are commit jobs as above whose output pointers are ignored: the
+ * server allocates leaf rows (if want_leaves), digests and caps on the device
+ * and keeps them until zkx_server_release.  Requests that arrive while others
+ * run are admitted as soon as an engine (stream) is free and their working
+ * set fits the budget next to everything admitted and not yet released. */
+typedef struct {
+    void *leaves, *digests, *caps;  /* device outputs (leaves NULL if not wanted) */
+    double t_submit, t_start, t_done; /* steady-clock seconds */
+    int state;                        /* 2 done, 3 failed */
+    int reserved;
+} ZkxServeResult;
+
+typedef struct {
+    uint64_t submitted, completed, failed, batches, max_batch;
+    size_t bytes_in_use, peak_bytes;
+} ZkxServeStats;
+
+RustError zkx_server_create(size_t device_id, size_t budget_bytes, int nstreams, size_t max_batch_jobs,
+                            void **server);
+RustError zkx_server_destroy(void *server);
+/* Device bytes a request occupies while admitted (scratch + outputs). */
+size_t zkx_server_footprint(const ZkxCommitJob *job, int want_leaves);
+RustError zkx_server_submit(void *server, const ZkxCommitJob *job, int want_leaves, uint64_t *id);
+RustError zkx_server_wait(void *server, uint64_t id, ZkxServeResult *result);
+RustError zkx_server_release(void *server, uint64_t id);
+RustError zkx_server_stats(void *server, ZkxServeStats *stats);
+
 /* Kernel timing: when enabled, every launch of the library's kernels on a
  * device is bracketed by HIP events on the launching stream; zkx_kernel_stats
  * reports, per kernel family, the launch count and summed duration. */

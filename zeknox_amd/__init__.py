@@ -84,6 +84,20 @@ class CommitJob(ctypes.Structure):
     ]
 
 
+class ServeResult(ctypes.Structure):
+    """zeknox_ext.h ZkxServeResult."""
+    _fields_ = [("leaves", ctypes.c_void_p), ("digests", ctypes.c_void_p), ("caps", ctypes.c_void_p),
+                ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("state", ctypes.c_int), ("reserved", ctypes.c_int)]
+
+
+class ServeStats(ctypes.Structure):
+    """zeknox_ext.h ZkxServeStats."""
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64),
+                ("batches", ctypes.c_uint64), ("max_batch", ctypes.c_uint64), ("bytes_in_use", ctypes.c_size_t),
+                ("peak_bytes", ctypes.c_size_t)]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run zeknox_amd.build.build() (hipcc, gfx950)")
@@ -118,6 +132,13 @@ def _load():
         "zkx_kernel_stats_reset": (None, []),
         "zkx_commit_batch": (RustError, [sz, ctypes.POINTER(CommitJob), sz]),
         "zkx_commit_scratch_bytes": (sz, [ctypes.POINTER(CommitJob), sz]),
+        "zkx_server_create": (RustError, [sz, sz, ctypes.c_int, sz, ctypes.POINTER(vp)]),
+        "zkx_server_destroy": (RustError, [vp]),
+        "zkx_server_footprint": (sz, [ctypes.POINTER(CommitJob), ctypes.c_int]),
+        "zkx_server_submit": (RustError, [vp, ctypes.POINTER(CommitJob), ctypes.c_int, ctypes.POINTER(u64)]),
+        "zkx_server_wait": (RustError, [vp, u64, ctypes.POINTER(ServeResult)]),
+        "zkx_server_release": (RustError, [vp, u64]),
+        "zkx_server_stats": (RustError, [vp, ctypes.POINTER(ServeStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -314,6 +335,68 @@ def commit_batch(commits, device_id=0):
 def commit_scratch_bytes(commits):
     arr = (CommitJob * len(commits))(*[c.job() for c in commits])
     return lib.zkx_commit_scratch_bytes(arr, len(commits))
+
+
+# ---- commit server -----------------------------------------------------------
+
+class Server:
+    """Continuous-batching commit server (zeknox_ext.h zkx_server_*)."""
+
+    def __init__(self, budget_bytes, streams=2, max_batch=64, device_id=0):
+        h = ctypes.c_void_p()
+        _check(lib.zkx_server_create(device_id, int(budget_bytes), streams, max_batch, ctypes.byref(h)))
+        self.h = h.value
+        self.device_id = device_id
+
+    def submit(self, coeffs, cols, lg_n, rate, cap_height, hash_type=HASH_POSEIDON, want_leaves=True):
+        """coeffs: DeviceBuffer (kept alive by the caller until done)."""
+        job = CommitJob(_addr(coeffs), None, None, None, cols, lg_n, rate, cap_height, hash_type, 0)
+        rid = ctypes.c_uint64()
+        _check(lib.zkx_server_submit(self.h, ctypes.byref(job), 1 if want_leaves else 0, ctypes.byref(rid)))
+        return rid.value
+
+    def wait(self, rid):
+        r = ServeResult()
+        _check(lib.zkx_server_wait(self.h, rid, ctypes.byref(r)))
+        return r
+
+    def results(self, r, cols, lg_n, rate, cap_height):
+        """Copy a finished request's outputs to the host: (leaves, digests, caps)."""
+        N = 1 << (lg_n + rate)
+        nc = 1 << cap_height
+        nd = 0 if nc == N else 2 * (N - nc)
+
+        def fetch(ptr, n):
+            out = np.empty(n, np.uint64)
+            if n:
+                _check(lib.zkx_memcpy_dtoh(self.device_id, out.ctypes.data, ptr, n * 8))
+            return out
+        leaves = fetch(r.leaves, cols * N) if r.leaves else None
+        return leaves, fetch(r.digests, nd * 4), fetch(r.caps, nc * 4)
+
+    def release(self, rid):
+        _check(lib.zkx_server_release(self.h, rid))
+
+    def stats(self):
+        st = ServeStats()
+        _check(lib.zkx_server_stats(self.h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in ServeStats._fields_}
+
+    @staticmethod
+    def footprint(cols, lg_n, rate, cap_height, want_leaves=True):
+        job = CommitJob(None, None, None, None, cols, lg_n, rate, cap_height, 0, 0)
+        return lib.zkx_server_footprint(ctypes.byref(job), 1 if want_leaves else 0)
+
+    def close(self):
+        if self.h:
+            _check(lib.zkx_server_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---- timing ---------------------------------------------------------------

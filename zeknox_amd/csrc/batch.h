@@ -46,8 +46,10 @@ public:
 
     // Enqueue the jobs on the engine's stream; returns without waiting.  The
     // tables the jobs need are built on first use.  Calls on one engine are
-    // serialised by `mu`.
-    void commit(const CommitJob *jobs, size_t n);
+    // serialised by `mu`.  `scratch` (>= commit_scratch_bytes, device,
+    // stream-ordered on this engine's stream) replaces the engine's own
+    // working buffer when given.
+    void commit(const CommitJob *jobs, size_t n, void *scratch = nullptr);
     std::mutex mu;
     void sync();
     hipStream_t stream() const { return stream_; }

@@ -59,7 +59,7 @@ Engine::~Engine()
 
 void Engine::sync() { ZKX_HIP(hipStreamSynchronize(stream_)); }
 
-void Engine::commit(const CommitJob *jobs, size_t n)
+void Engine::commit(const CommitJob *jobs, size_t n, void *scratch)
 {
     if (n == 0) return;
     for (size_t i = 0; i < n; i++) check_commit_job(jobs[i]);
@@ -67,15 +67,16 @@ void Engine::commit(const CommitJob *jobs, size_t n)
 
     // ---- working memory: every job's LDE in DIF (bit-reversed) order
     const size_t need = commit_scratch_bytes(jobs, n);
-    if (scratch_.bytes < need) {
+    if (!scratch && scratch_.bytes < need) {
         sync();
         scratch_ = DevBuf(need);
     }
+    char *base = static_cast<char *>(scratch ? scratch : scratch_.p);
     std::vector<uint64_t *> lde(n);
     {
         size_t off = 0;
         for (size_t i = 0; i < n; i++) {
-            lde[i] = reinterpret_cast<uint64_t *>(static_cast<char *>(scratch_.p) + off);
+            lde[i] = reinterpret_cast<uint64_t *>(base + off);
             off += align_up(lde_elems(jobs[i]) * sizeof(uint64_t));
         }
     }

@@ -327,3 +327,91 @@ size_t zkx_commit_scratch_bytes(const ZkxCommitJob *jobs, size_t njobs)
 }
 
 }  // extern "C"
+
+// ---- commit server (zeknox_ext.h) -------------------------------------------
+
+#include "serve.h"
+
+extern "C" {
+
+RustError zkx_server_create(size_t device_id, size_t budget_bytes, int nstreams, size_t max_batch_jobs,
+                            void **server)
+{
+    return guarded([&] {
+        if (!server) invalid("null server handle");
+        Device &d = device(device_id);
+        *server = new Server(d, budget_bytes, nstreams, max_batch_jobs);
+    });
+}
+
+RustError zkx_server_destroy(void *server)
+{
+    return guarded([&] { delete static_cast<Server *>(server); });
+}
+
+size_t zkx_server_footprint(const ZkxCommitJob *job, int want_leaves)
+{
+    try {
+        std::vector<CommitJob> v = to_jobs(job, 1);
+        return Server::footprint(v[0], want_leaves != 0);
+    } catch (...) {
+        return 0;
+    }
+}
+
+RustError zkx_server_submit(void *server, const ZkxCommitJob *job, int want_leaves, uint64_t *id)
+{
+    return guarded([&] {
+        if (!server || !id) invalid("null argument");
+        std::vector<CommitJob> v = to_jobs(job, 1);
+        v[0].leaves = v[0].digests = v[0].caps = nullptr;
+        // output pointers are the server's; a placeholder passes the job check
+        v[0].caps = reinterpret_cast<uint64_t *>(1);
+        v[0].digests = reinterpret_cast<uint64_t *>(1);
+        *id = static_cast<Server *>(server)->submit(v[0], want_leaves != 0);
+    });
+}
+
+RustError zkx_server_wait(void *server, uint64_t id, ZkxServeResult *result)
+{
+    return guarded([&] {
+        if (!server) invalid("null server");
+        ServeRequest r = static_cast<Server *>(server)->wait(id);
+        if (result) {
+            result->leaves = r.job.leaves;
+            result->digests = r.job.digests;
+            result->caps = r.job.caps;
+            result->t_submit = r.t_submit;
+            result->t_start = r.t_start;
+            result->t_done = r.t_done;
+            result->state = r.state;
+            result->reserved = 0;
+        }
+        if (r.state == 3) throw Error(-1, "request failed: " + r.error);
+    });
+}
+
+RustError zkx_server_release(void *server, uint64_t id)
+{
+    return guarded([&] {
+        if (!server) invalid("null server");
+        static_cast<Server *>(server)->release(id);
+    });
+}
+
+RustError zkx_server_stats(void *server, ZkxServeStats *stats)
+{
+    return guarded([&] {
+        if (!server || !stats) invalid("null argument");
+        ServeStats s = static_cast<Server *>(server)->stats();
+        stats->submitted = s.submitted;
+        stats->completed = s.completed;
+        stats->failed = s.failed;
+        stats->batches = s.batches;
+        stats->max_batch = s.max_batch;
+        stats->bytes_in_use = s.bytes_in_use;
+        stats->peak_bytes = s.peak_bytes;
+    });
+}
+
+}  // extern "C"

@@ -27,6 +27,20 @@ __device__ __forceinline__ uint32_t lo32(uint64_t x) { return (uint32_t)x; }
 __device__ __forceinline__ uint32_t hi32(uint64_t x) { return (uint32_t)(x >> 32); }
 __device__ __forceinline__ uint64_t pack(uint32_t lo, uint32_t hi) { return ((uint64_t)hi << 32) | lo; }
 
+// A wave-uniform constant the compiler cannot see through: it stays an SGPR
+// operand of v_mad_u64_u32 instead of being strength-reduced into 64-bit
+// shift-adds (which issue at about a third of the multiply-add rate).
+__device__ __forceinline__ uint32_t opaque(uint32_t c)
+{
+    uint32_t r;
+    asm("; opaque %0" : "=s"(r) : "0"(c));
+    return r;
+}
+
+// x + y for y < 2^32 as one full-rate v_mad_u64_u32 (y * 1 + x); valid when
+// the sum fits in 64 bits.
+__device__ __forceinline__ uint64_t add_small(uint64_t x, uint32_t y) { return (uint64_t)y * opaque(1u) + x; }
+
 // (a + b) mod p for a < 2^64 and b < p, canonical when either
 //   a < p, or b <= 2^64 - 2^33 + 1 (then a may be any u64).
 // s = a + b; t = s + EPS = s - p (mod 2^64); a + b >= p iff either carries.

@@ -30,7 +30,7 @@ static __constant__ uint64_t c_poseidon2_rc[30 * 12] = {ZK_POSEIDON2_RC_LIST};
 // middle word is B + hi(A) formed with one v_mad_u64_u32 (x * 1 + y).
 __device__ __forceinline__ uint64_t limb_combine(uint64_t A, uint64_t B)
 {
-    const uint64_t T = B + (uint64_t)gl::hi32(A);  // < 2^64
+    const uint64_t T = gl::add_small(B, gl::hi32(A));  // < 2^64
     return gl::reduce96(gl::pack(gl::lo32(A), gl::lo32(T)), gl::hi32(T));
 }
 
@@ -54,7 +54,10 @@ __device__ __forceinline__ void poseidon_mds(uint64_t s[12], const uint64_t *k)
 #pragma unroll
         for (int i = 0; i < 12; i++) {
             const int j = (i + r) % 12;
-            const uint32_t c = (uint32_t)ZK_POSEIDON_MDS_CIRC[i] + (i == 0 ? (uint32_t)ZK_POSEIDON_MDS_DIAG[r] : 0u);
+            const uint32_t c0 = (uint32_t)ZK_POSEIDON_MDS_CIRC[i] + (i == 0 ? (uint32_t)ZK_POSEIDON_MDS_DIAG[r] : 0u);
+            // the first product takes the SGPR constant pair as its addend, so
+            // its coefficient stays an inline constant (one SGPR per VALU op)
+            const uint32_t c = i == 0 ? c0 : gl::opaque(c0);
             A = (uint64_t)lo[j] * c + A;
             B = (uint64_t)hi[j] * c + B;
         }
@@ -102,8 +105,8 @@ __device__ __forceinline__ void p2_external(uint64_t s[12])
         for (int c = 0; c < 4; c++)
 #pragma unroll
             for (int b = 0; b < 12; b += 4) {
-                A = (uint64_t)lo[b + c] * M4[r][c] + A;
-                B = (uint64_t)hi[b + c] * M4[r][c] + B;
+                A = (uint64_t)lo[b + c] * gl::opaque(M4[r][c]) + A;
+                B = (uint64_t)hi[b + c] * gl::opaque(M4[r][c]) + B;
             }
         VA[r] = A;
         VB[r] = B;
@@ -115,8 +118,8 @@ __device__ __forceinline__ void p2_external(uint64_t s[12])
             uint64_t A = VA[r], B = VB[r];
 #pragma unroll
             for (int c = 0; c < 4; c++) {
-                A = (uint64_t)lo[b + c] * M4[r][c] + A;
-                B = (uint64_t)hi[b + c] * M4[r][c] + B;
+                A = (uint64_t)lo[b + c] * gl::opaque(M4[r][c]) + A;
+                B = (uint64_t)hi[b + c] * gl::opaque(M4[r][c]) + B;
             }
             s[b + r] = limb_combine(A, B);
         }
