@@ -85,9 +85,11 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch)
     // stream is ordered after: the null-stream-blocking device stream is
     // synchronised before first use)
     std::vector<const uint64_t *> tw(n), F(n);
+    const uint64_t *tw12 = nullptr;
     {
         std::lock_guard<std::mutex> g(dev_.mu);
-        bool built = false;
+        bool built = dev_.twiddles[kNttTileLog] == nullptr;
+        tw12 = dev_.twiddle_table(kNttTileLog);
         for (size_t i = 0; i < n; i++) {
             const uint32_t lg = jobs[i].lg_n + jobs[i].rate;
             built |= dev_.twiddles[lg] == nullptr;
@@ -107,8 +109,8 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch)
     for (size_t i = 0; i < n; i++) {
         const CommitJob &j = jobs[i];
         const uint32_t lg = j.lg_n + j.rate;
-        ntt_tiles[i] = plan_dif(ntt_plan[i], lde[i], (uint64_t)j.cols << j.rate, j.lg_n, lg, tw[i], false, kSrcLde,
-                                j.coeffs, j.rate, false, F[i]);
+        ntt_tiles[i] = plan_dif(ntt_plan[i], lde[i], (uint64_t)j.cols << j.rate, j.lg_n, lg, tw[i], tw12, false,
+                                kSrcLde, j.coeffs, j.rate, false, F[i]);
         MerkleSpec s;
         s.digests = j.digests;
         s.caps = j.caps;
@@ -146,20 +148,22 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch)
         memcpy(host.data() + off, p, bytes);
         return off;
     };
-    for (size_t k = 0; k < max_ntt; k++) {
-        std::vector<NttPass> list;
-        uint64_t blocks = 0;
-        for (size_t i = 0; i < n; i++) {
-            if (k >= ntt_plan[i].size()) continue;
-            NttPass p = ntt_plan[i][k];
-            p.tile_begin = blocks;
-            blocks += ntt_tiles[i];
-            list.push_back(p);
+    for (size_t k = 0; k < max_ntt; k++)
+        for (int contig = 0; contig < 2; contig++) {
+            std::vector<NttPass> list;
+            uint64_t blocks = 0;
+            for (size_t i = 0; i < n; i++) {
+                if (k >= ntt_plan[i].size() || (ntt_plan[i][k].s_lo == 0) != (contig == 1)) continue;
+                NttPass p = ntt_plan[i][k];
+                p.tile_begin = blocks;
+                blocks += ntt_tiles[i];
+                list.push_back(p);
+            }
+            if (list.empty()) continue;
+            const size_t off = push(list.data(), list.size() * sizeof(NttPass));
+            launches.push_back({off, (uint32_t)list.size(), blocks, 0, contig, false});
+            host.resize(align_up(host.size()));
         }
-        const size_t off = push(list.data(), list.size() * sizeof(NttPass));
-        launches.push_back({off, (uint32_t)list.size(), blocks, 0, 0, false});
-        host.resize(align_up(host.size()));
-    }
     for (size_t k = 0; k < max_mk; k++)
         for (int ht : {0, 3}) {
             std::vector<MerkleArgs> list;
@@ -184,7 +188,8 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch)
     for (const Launch &l : launches) {
         const char *base = static_cast<const char *>(desc_.p) + l.offset;
         if (l.kind == 0) {
-            launch_ntt_list(stream_, reinterpret_cast<const NttPass *>(base), l.count, l.blocks, "batch_ntt");
+            launch_ntt_list(stream_, reinterpret_cast<const NttPass *>(base), l.count, l.blocks, l.hash == 1,
+                            "batch_ntt");
         } else {
             launch_merkle_pass(stream_, l.hash, l.leaves, reinterpret_cast<const MerkleArgs *>(base), l.count,
                                MerkleArgs{}, l.blocks);

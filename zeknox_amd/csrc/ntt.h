@@ -30,11 +30,13 @@ void transpose_rev(Device &d, uint64_t *out, const uint64_t *in, uint32_t lg, ui
 // order; each pass must finish before the next starts).  Returns the tile
 // count of every pass.
 uint64_t plan_dif(std::vector<NttPass> &out, uint64_t *buf, uint64_t nblk, uint32_t L, uint32_t lg_tw,
-                  const uint64_t *tw, bool inverse, uint32_t mode0, const uint64_t *src, uint32_t rate,
-                  bool with_coset, const uint64_t *coset);
+                  const uint64_t *tw, const uint64_t *tw12, bool inverse, uint32_t mode0, const uint64_t *src,
+                  uint32_t rate, bool with_coset, const uint64_t *coset);
 
-// One launch over n pass descriptors in device memory (sorted by tile_begin).
-void launch_ntt_list(hipStream_t stream, const NttPass *dev_list, uint32_t n, uint64_t blocks, const char *name);
+// One launch over n pass descriptors in device memory (sorted by
+// tile_begin), all contiguous (s_lo == 0) or all strided.
+void launch_ntt_list(hipStream_t stream, const NttPass *dev_list, uint32_t n, uint64_t blocks, bool contiguous,
+                     const char *name);
 
 // table[i] = base^i, i < n.
 void fill_powers(uint64_t *dst, uint64_t n, uint64_t base, hipStream_t s);

@@ -18,8 +18,12 @@ void launch_pass(Device &d, const NttPass &p, uint64_t tiles, const char *name)
 {
     if (tiles > 0x7FFFFFFFull) invalid("transform too large for one launch");
     ProfScope ps(name, d.stream);
-    hipLaunchKernelGGL(k_ntt_tiles, dim3((uint32_t)tiles), dim3(kNttThreads), 0, d.stream,
-                       (const NttPass *)nullptr, 0u, p);
+    if (p.s_lo == 0)
+        hipLaunchKernelGGL(k_ntt_tiles<true>, dim3((uint32_t)tiles), dim3(kNttThreads), 0, d.stream,
+                           (const NttPass *)nullptr, 0u, p);
+    else
+        hipLaunchKernelGGL(k_ntt_tiles<false>, dim3((uint32_t)tiles), dim3(kNttThreads), 0, d.stream,
+                           (const NttPass *)nullptr, 0u, p);
     ZKX_HIP(hipGetLastError());
 }
 
@@ -29,24 +33,31 @@ void run_dif(Device &d, uint64_t *buf, uint64_t nblk, uint32_t L, uint32_t lg_tw
              uint32_t mode0, const uint64_t *src, uint32_t rate, bool with_coset, const uint64_t *coset)
 {
     std::vector<NttPass> passes;
-    uint64_t tiles = plan_dif(passes, buf, nblk, L, lg_tw, tw, inverse, mode0, src, rate, with_coset, coset);
+    const uint64_t *tw12 = d.twiddle_table(kNttTileLog);
+    uint64_t tiles = plan_dif(passes, buf, nblk, L, lg_tw, tw, tw12, inverse, mode0, src, rate, with_coset, coset);
     for (const NttPass &p : passes) launch_pass(d, p, tiles, p.mode == kSrcLde ? "ntt_pass_lde" : "ntt_pass");
 }
 
 }  // namespace
 
-void launch_ntt_list(hipStream_t stream, const NttPass *dev_list, uint32_t n, uint64_t blocks, const char *name)
+void launch_ntt_list(hipStream_t stream, const NttPass *dev_list, uint32_t n, uint64_t blocks, bool contiguous,
+                     const char *name)
 {
     if (blocks == 0) return;
     if (blocks > 0x7FFFFFFFull) invalid("transform batch too large for one launch");
     ProfScope ps(name, stream);
-    hipLaunchKernelGGL(k_ntt_tiles, dim3((uint32_t)blocks), dim3(kNttThreads), 0, stream, dev_list, n, NttPass{});
+    if (contiguous)
+        hipLaunchKernelGGL(k_ntt_tiles<true>, dim3((uint32_t)blocks), dim3(kNttThreads), 0, stream, dev_list, n,
+                           NttPass{});
+    else
+        hipLaunchKernelGGL(k_ntt_tiles<false>, dim3((uint32_t)blocks), dim3(kNttThreads), 0, stream, dev_list, n,
+                           NttPass{});
     ZKX_HIP(hipGetLastError());
 }
 
 uint64_t plan_dif(std::vector<NttPass> &out, uint64_t *buf, uint64_t nblk, uint32_t L, uint32_t lg_tw,
-                  const uint64_t *tw, bool inverse, uint32_t mode0, const uint64_t *src, uint32_t rate,
-                  bool with_coset, const uint64_t *coset)
+                  const uint64_t *tw, const uint64_t *tw12, bool inverse, uint32_t mode0, const uint64_t *src,
+                  uint32_t rate, bool with_coset, const uint64_t *coset)
 {
     const uint64_t total = nblk << L;
     const uint64_t tiles = (total + (1ull << kNttTileLog) - 1) >> kNttTileLog;
@@ -58,6 +69,7 @@ uint64_t plan_dif(std::vector<NttPass> &out, uint64_t *buf, uint64_t nblk, uint3
         a.src = src;
         a.tw = tw;
         a.coset = coset;
+        a.tw12 = tw12;
         a.total = total;
         a.L = L;
         a.lg_tw = lg_tw;

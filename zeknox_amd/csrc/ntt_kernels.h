@@ -59,9 +59,9 @@ __device__ __forceinline__ uint64_t tile_index(const NttPass &a, uint64_t tile, 
 // 2^(4-K) groups taken from the extra element bits; the other u-bits come
 // from the lane id.  The twiddle of stage d for element e depends only on
 // (j mod 2^d) << s_lo | lo.
-template <int K>
-__device__ __forceinline__ void ntt_round(uint64_t *lds, const NttPass &a, uint32_t t, uint32_t b0, uint32_t jpos,
-                                          uint64_t lo_base)
+template <bool CONTIG, int K>
+__device__ __forceinline__ void ntt_round(uint64_t *lds, const uint64_t *twl, const NttPass &a, uint32_t t,
+                                          uint32_t b0, uint32_t jpos, uint64_t lo_base)
 {
     constexpr int E = kNttPerThread;
     const uint32_t RB = jpos + b0;
@@ -84,12 +84,18 @@ __device__ __forceinline__ void ntt_round(uint64_t *lds, const NttPass &a, uint3
             if ((e >> dd) & 1) continue;
             const int f = e | (1 << dd);
             const uint32_t u = uu[e];
-            const uint32_t j = a.s_lo > 0 ? (u >> a.c) : (u & ((1u << a.m) - 1));
-            const uint64_t lo_part = a.s_lo > 0 ? lo_base + (u & ((1u << a.c) - 1)) : 0;
-            const uint64_t ex = ((((uint64_t)(j & ((1u << d) - 1))) << a.s_lo) | lo_part) << shift;
+            const uint32_t j = !CONTIG ? (u >> a.c) : (u & ((1u << a.m) - 1));
+            uint64_t w;
+            if constexpr (CONTIG) {
+                w = twl[(j & ((1u << d) - 1)) << (kNttTileLog - 1 - d)];
+            } else {
+                const uint64_t lo_part = lo_base + (u & ((1u << a.c) - 1));
+                const uint64_t ex = ((((uint64_t)(j & ((1u << d) - 1))) << a.s_lo) | lo_part) << shift;
+                w = tw_at(a.tw, a.lg_tw, ex, a.inverse);
+            }
             const uint64_t y0 = x[e], y1 = x[f];
             x[e] = gl::add(y0, y1);
-            x[f] = gl::mul(gl::sub(y0, y1), tw_at(a.tw, a.lg_tw, ex, a.inverse));
+            x[f] = gl::mul(gl::sub(y0, y1), w);
         }
     }
 #pragma unroll
@@ -98,8 +104,9 @@ __device__ __forceinline__ void ntt_round(uint64_t *lds, const NttPass &a, uint3
 
 // Full-width round (K = 4, no extra element bits): the lane's other u-bits
 // are its id, so the 15 distinct twiddles of the round are loaded once.
-__device__ __forceinline__ void ntt_round4(uint64_t *lds, const NttPass &a, uint32_t t, uint32_t b0, uint32_t jpos,
-                                           uint64_t lo_base)
+template <bool CONTIG>
+__device__ __forceinline__ void ntt_round4(uint64_t *lds, const uint64_t *twl, const NttPass &a, uint32_t t,
+                                           uint32_t b0, uint32_t jpos, uint64_t lo_base)
 {
     constexpr int E = kNttPerThread;
     const uint32_t RB = jpos + b0;
@@ -108,17 +115,23 @@ __device__ __forceinline__ void ntt_round4(uint64_t *lds, const NttPass &a, uint
 #pragma unroll
     for (int e = 0; e < E; e++) x[e] = lds[lds_slot(ubase | ((uint32_t)e << RB))];
     // lane constants: j-bits below b0 and the column's low in-block bits
-    const uint32_t j0 = (a.s_lo > 0 ? (ubase >> a.c) : (ubase & ((1u << a.m) - 1))) & ((1u << b0) - 1);
-    const uint64_t lo_part = a.s_lo > 0 ? lo_base + (ubase & ((1u << a.c) - 1)) : 0;
+    const uint32_t j0 = (!CONTIG ? (ubase >> a.c) : (ubase & ((1u << a.m) - 1))) & ((1u << b0) - 1);
+    const uint64_t lo_part = !CONTIG ? lo_base + (ubase & ((1u << a.c) - 1)) : 0;
     const uint64_t base = ((uint64_t)j0 << a.s_lo) | lo_part;
 #pragma unroll
     for (int dd = 3; dd >= 0; dd--) {
         const uint32_t d = b0 + (uint32_t)dd;
         const uint32_t shift = a.lg_tw - (a.s_lo + d) - 1;
         uint64_t w[1 << dd];
+        if constexpr (CONTIG) {
+            // contiguous pass: w_{2^(d+1)}^x = w_4096^(x << (11 - d)) from LDS
 #pragma unroll
-        for (int v = 0; v < (1 << dd); v++)
-            w[v] = tw_at(a.tw, a.lg_tw, (base + ((uint64_t)v << (b0 + a.s_lo))) << shift, a.inverse);
+            for (int v = 0; v < (1 << dd); v++) w[v] = twl[(j0 | ((uint32_t)v << b0)) << (kNttTileLog - 1 - d)];
+        } else {
+#pragma unroll
+            for (int v = 0; v < (1 << dd); v++)
+                w[v] = tw_at(a.tw, a.lg_tw, (base + ((uint64_t)v << (b0 + a.s_lo))) << shift, a.inverse);
+        }
 #pragma unroll
         for (int e = 0; e < E; e++) {
             if ((e >> dd) & 1) continue;
@@ -134,10 +147,13 @@ __device__ __forceinline__ void ntt_round4(uint64_t *lds, const NttPass &a, uint
 
 // `passes` (npasses > 0, device memory) lists independent passes sorted by
 // tile_begin; with npasses == 0 the launch runs the single pass `one`.
-__global__ void __launch_bounds__(kNttThreads) k_ntt_tiles(const NttPass *__restrict__ passes, uint32_t npasses,
+// CONTIG: every pass of the launch is contiguous (s_lo == 0), else strided.
+template <bool CONTIG>
+__global__ void __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_eu(3))) k_ntt_tiles(const NttPass *__restrict__ passes, uint32_t npasses,
                                                          NttPass one)
 {
     __shared__ uint64_t lds[(1 << kNttTileLog) + (1 << kNttTileLog) / 32];
+    __shared__ uint64_t twl[1 << (kNttTileLog - 1)];  // w_4096^k (or ^-k), k < 2048
 
     // locate this work-group's pass (uniform)
     uint32_t lo = 0, hi = npasses;
@@ -150,8 +166,9 @@ __global__ void __launch_bounds__(kNttThreads) k_ntt_tiles(const NttPass *__rest
     const uint64_t tile = blockIdx.x - a.tile_begin;
     const uint32_t t = threadIdx.x;
 
-    // ---- load, coalesced: lane t takes positions t + 256 i
-#pragma unroll
+    // ---- load, coalesced: lane t takes positions t + 256 i (4 at a time in
+    // flight keeps the address and data registers below the round's needs)
+#pragma unroll 4
     for (int i = 0; i < kNttPerThread; i++) {
         const uint32_t u = t + kNttThreads * i;
         const uint64_t idx = tile_index(a, tile, u);
@@ -170,26 +187,34 @@ __global__ void __launch_bounds__(kNttThreads) k_ntt_tiles(const NttPass *__rest
         }
         lds[lds_slot(u)] = v;
     }
+    // contiguous passes take every stage twiddle from one LDS table
+    if (CONTIG && a.m > 0) {
+#pragma unroll
+        for (int i = 0; i < (1 << (kNttTileLog - 1)) / kNttThreads; i++) {
+            const uint32_t k = t + kNttThreads * i;
+            twl[k] = a.tw12[a.inverse ? ((0u - k) & ((1u << kNttTileLog) - 1)) : k];
+        }
+    }
     __syncthreads();
 
     // low in-block bits shared by the tile's columns (strided passes)
-    const uint64_t lo_base = a.s_lo > 0 ? ((tile & ((1ull << (a.s_lo - a.c)) - 1)) << a.c) : 0;
-    const uint32_t jpos = a.s_lo > 0 ? a.c : 0;  // u-bit of j-bit 0
+    const uint64_t lo_base = !CONTIG ? ((tile & ((1ull << (a.s_lo - a.c)) - 1)) << a.c) : 0;
+    const uint32_t jpos = !CONTIG ? a.c : 0;  // u-bit of j-bit 0
 
     // ---- register rounds, top j-bits first: 4 stages each, then the rest
     uint32_t rem = a.m;
     while (rem >= 4) {
         rem -= 4;
-        ntt_round4(lds, a, t, rem, jpos, lo_base);
+        ntt_round4<CONTIG>(lds, twl, a, t, rem, jpos, lo_base);
         __syncthreads();
     }
-    if (rem == 3) ntt_round<3>(lds, a, t, 0, jpos, lo_base);
-    else if (rem == 2) ntt_round<2>(lds, a, t, 0, jpos, lo_base);
-    else if (rem == 1) ntt_round<1>(lds, a, t, 0, jpos, lo_base);
+    if (rem == 3) ntt_round<CONTIG, 3>(lds, twl, a, t, 0, jpos, lo_base);
+    else if (rem == 2) ntt_round<CONTIG, 2>(lds, twl, a, t, 0, jpos, lo_base);
+    else if (rem == 1) ntt_round<CONTIG, 1>(lds, twl, a, t, 0, jpos, lo_base);
     if (rem) __syncthreads();
 
     // ---- store, coalesced
-#pragma unroll
+#pragma unroll 4
     for (int i = 0; i < kNttPerThread; i++) {
         const uint32_t u = t + kNttThreads * i;
         const uint64_t idx = tile_index(a, tile, u);

@@ -26,6 +26,7 @@ struct NttPass {
     const uint64_t *src;    // kSrcLde: coefficients [cols][2^L]
     const uint64_t *tw;     // w_T^k, k < 2^lg_tw (w_T = OMEGA[lg_tw])
     const uint64_t *coset;  // kSrcCoset: g^i table; kSrcLde: factor table F (see ntt_kernels.h)
+    const uint64_t *tw12;   // w_4096^k, k < 4096 (stage twiddles of contiguous passes)
     uint64_t total;         // elements in buf
     uint64_t tile_begin;    // first work-group of this pass in the launch
     uint32_t L;             // log2 block length
