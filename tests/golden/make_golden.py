@@ -1,0 +1,93 @@
+#!/usr/bin/env python3
+"""Write tests/golden/fixtures.json: expected outputs for the GPU and oracle
+parity tests.
+
+Merkle fixtures come from the reference's OWN CPU code
+(native/merkle/merkle.cpp fill_digests_buf_linear_cpu with its Poseidon and
+Poseidon2 hashers), compiled from its sources by oracle/Makefile into
+oracle/_ref/libzkref.so; this script needs that build (i.e. the reference
+tree) and is run once in the build container:
+
+    make -C oracle ref && python3 tests/golden/make_golden.py
+
+LDE / transpose fixtures come from the oracle (the reference has no CPU LDE);
+the oracle's LDE is pinned by the brute-force DFT and closed-form tests.
+
+Inputs are not stored: they are regenerated from (count, seed) by the shared
+splitmix64 generator (oracle orc_fill_canonical), so the fixture file stays
+small.  Small outputs are stored in full; large ones as SHA-256 of the
+little-endian u64 bytes.
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+import oracle_bind as ob  # noqa: E402
+
+MERKLE = [  # (log2 leaves, leaf size, cap height, hash, seed)
+    (3, 7, 3, 0, 11), (3, 7, 2, 0, 12), (4, 7, 1, 0, 13), (5, 9, 2, 3, 14), (6, 2, 4, 0, 15),
+    (6, 4, 0, 3, 16), (7, 5, 3, 0, 17), (8, 8, 4, 3, 18), (8, 20, 4, 0, 19), (9, 135, 4, 3, 20),
+    (10, 80, 4, 0, 21), (10, 1, 2, 3, 22), (11, 13, 9, 0, 23), (12, 135, 4, 0, 24),
+]
+LARGE = [  # hashed outputs: 2^13 x 135, cap height 4, both hashes
+    (13, 135, 4, 0, 1001), (13, 135, 4, 3, 1002),
+]
+LDE = [  # (lg_n, rate, cols, coset, seed)
+    (4, 1, 3, True, 31), (6, 2, 2, True, 32), (8, 3, 4, False, 33), (10, 1, 5, True, 34),
+]
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<u8").tobytes()).hexdigest()
+
+
+def ref_merkle(ref, leaves, ls, h, ht):
+    nl = leaves.size // ls
+    nc = 1 << h
+    nd = 2 * (nl - nc) if nl != nc else 0
+    d = np.zeros(max(nd, 1) * 4, np.uint64)
+    c = np.zeros(nc * 4, np.uint64)
+    ref.fill_digests_buf_linear_cpu(ob._ptr(d), ob._ptr(c), ob._ptr(leaves), nd, nc, nl, ls, h, ht)
+    return d[:nd * 4], c
+
+
+def main():
+    ref = ob.ref_lib()
+    if ref is None:
+        raise SystemExit("oracle/_ref/libzkref.so is not built (make -C oracle ref)")
+    out = {"generator": "splitmix64 rejection-sampled into [0, p): oracle orc_fill_canonical(count, seed)",
+           "merkle_source": "reference native/merkle/merkle.cpp (compiled from its sources)",
+           "lde_source": "oracle/oracle.c orc_lde + orc_transpose_rev",
+           "merkle": [], "merkle_sha256": [], "lde": []}
+    for lg, ls, h, ht, seed in MERKLE:
+        leaves = ob.canonical(ls << lg, seed)
+        d, c = ref_merkle(ref, leaves, ls, h, ht)
+        rec = {"lg_leaves": lg, "leaf_size": ls, "cap_height": h, "hash": ht, "seed": seed,
+               "caps": [str(int(v)) for v in c], "digests_sha256": sha(d)}
+        if d.size <= 256:
+            rec["digests"] = [str(int(v)) for v in d]
+        out["merkle"].append(rec)
+    for lg, ls, h, ht, seed in LARGE:
+        leaves = ob.canonical(ls << lg, seed)
+        d, c = ref_merkle(ref, leaves, ls, h, ht)
+        out["merkle_sha256"].append({"lg_leaves": lg, "leaf_size": ls, "cap_height": h, "hash": ht, "seed": seed,
+                                     "digests_sha256": sha(d), "caps": [str(int(v)) for v in c]})
+    for lg_n, rate, cols, coset, seed in LDE:
+        x = ob.canonical(cols << lg_n, seed)
+        lde = ob.lde(x, lg_n, rate, cols, coset=coset)
+        leaves = ob.transpose_rev(lde, lg_n + rate, cols)
+        out["lde"].append({"lg_n": lg_n, "rate": rate, "cols": cols, "coset": coset, "seed": seed,
+                           "lde_sha256": sha(lde), "leaves_sha256": sha(leaves),
+                           "lde_head": [str(int(v)) for v in lde[:8]]})
+    with open(os.path.join(HERE, "fixtures.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print("wrote", os.path.join(HERE, "fixtures.json"))
+
+
+if __name__ == "__main__":
+    main()
