@@ -50,9 +50,148 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="served-stream duration (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=40.0, help="served-stream arrivals per second")
-    ap.add_argument("--cpu-rows", type=int, default=1 << 13,
-                    help="leaf rows of the bounded CPU sample (of 2^18)")
+    ap.add_argument("--no-extra", action="store_true", help="skip the 64-job and 8 x 400 x 2^20 batches")
+    ap.add_argument("--extra-steps", type=int, default=3, help="timed steps of those batches")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="whole CPU commits timed (median reported)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain elementwise copy of MIB MiB three times (scale for HBM counters)")
     return ap.parse_args()
+
+
+def cpu_baseline(np, coeffs, runs, gpu_caps):
+    """The same config-2 commit on the host cores, `runs` times, median
+    reported.  Merkle: the reference's own CPU fill_digests_buf_linear_cpu
+    (native/merkle/merkle.cpp, OpenMP, compiled from its sources into
+    oracle/_ref) when that build is present, else the oracle's restatement of
+    it; LDE + transpose: the oracle (the reference has no CPU LDE).  The CPU
+    caps are compared with the GPU's."""
+    import statistics
+    import oracle_bind as ob
+    ref = ob.ref_lib()
+    nc = 1 << CAP_H
+    nd = 2 * (N - nc)
+    lde_t, mer_t, same = [], [], True
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        lde = ob.lde(coeffs, LG_N, RATE, COLS)
+        leaves = ob.transpose_rev(lde, LG, COLS)
+        del lde
+        t1 = time.perf_counter()
+        if ref is not None:
+            dig = np.zeros(nd * 4, np.uint64)
+            cap = np.zeros(nc * 4, np.uint64)
+            ref.fill_digests_buf_linear_cpu(ob._ptr(dig), ob._ptr(cap), ob._ptr(leaves), nd, nc, N, COLS, CAP_H, HASH)
+        else:
+            dig, cap = ob.merkle(leaves, COLS, CAP_H, HASH)
+        t2 = time.perf_counter()
+        lde_t.append(t1 - t0)
+        mer_t.append(t2 - t1)
+        same = same and bool(np.array_equal(cap, gpu_caps))
+        del leaves, dig
+    tot = [a + b for a, b in zip(lde_t, mer_t)]
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu_model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
+    return {"value": round(1.0 / statistics.median(tot), 4), "unit": "commits/s",
+            "cores": ob.orc.orc_num_threads(), "kind": "reference" if ref is not None else "port",
+            "sample": f"{runs} whole config-2 commits of the GPU run's input, median: LDE+transpose "
+                      f"{statistics.median(lde_t):.2f} s (oracle C, OpenMP), Poseidon Merkle "
+                      f"{statistics.median(mer_t):.2f} s ("
+                      + ("reference merkle.cpp compiled from its sources" if ref is not None else "oracle C")
+                      + ")",
+            "caps_equal_gpu": same, "nproc": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "cpu_model": cpu_model, "runs_s": [round(t, 3) for t in tot]}
+
+
+def _timed(zk, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    zk.lib.zkx_device_synchronize(0)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    zk.lib.zkx_device_synchronize(0)
+    return (time.perf_counter() - t0) / steps
+
+
+def plain_vs_batched(zk, np, shapes, steps, warmup, seed):
+    """Commit jobs of the given (cols, lg_n, rate, hash) shapes, cap height 4,
+    coefficients generated on the device: the plain C-ABI sequence one job
+    per call (shared LDE / leaf buffers) against one zkx_commit_batch of all
+    of them (every job's inputs, LDE and outputs resident at once).  Returns
+    commits/s both ways and whether every digest and cap agrees."""
+    commits = []
+    for j, (cols, lg_n, rate, ht) in enumerate(shapes):
+        d = zk.DeviceBuffer(cols << lg_n)
+        d.fill_random(seed * 1000 + j)
+        commits.append(zk.Commit(d, cols, lg_n, rate, CAP_H, ht))
+    big = max(c.cols * c.N for c in commits)
+    d_lde, d_leaves = zk.DeviceBuffer(big), zk.DeviceBuffer(big)
+    p_dig = [zk.DeviceBuffer(max(c.num_digests, 1) * 4) for c in commits]
+    p_cap = [zk.DeviceBuffer(c.num_caps * 4) for c in commits]
+
+    def plain():
+        for c, dg, cp in zip(commits, p_dig, p_cap):
+            lg = c.lg_n + c.rate
+            zk.lde_batch(0, d_lde, c.d_coeffs, c.lg_n,
+                         zk.NTTConfig(batches=c.cols, extension_rate_bits=c.rate, with_coset=True,
+                                      are_inputs_on_device=True, are_outputs_on_device=True))
+            zk.transpose_rev_batch(0, d_leaves, d_lde, lg, zk.TransposeConfig(batches=c.cols,
+                                   are_inputs_on_device=True, are_outputs_on_device=True))
+            zk.fill_digests_buf_linear_gpu_with_gpu_ptr(dg, cp, d_leaves, c.num_digests, c.num_caps, c.N, c.cols,
+                                                       CAP_H, c.hash_type, 0)
+
+    plain_s = _timed(zk, plain, steps, warmup)
+    batch_s = _timed(zk, lambda: zk.commit_batch(commits), steps, warmup)
+    same = all(np.array_equal(cp.to_host(), c.d_caps.to_host())
+               and np.array_equal(dg.to_host(c.num_digests * 4), c.d_digests.to_host(c.num_digests * 4))
+               for c, dg, cp in zip(commits, p_dig, p_cap))
+    n = len(commits)
+    for b in [d_lde, d_leaves] + p_dig + p_cap:
+        b.free()
+    for c in commits:
+        for b in (c.d_coeffs, c.d_leaves, c.d_digests, c.d_caps):
+            if b is not None:
+                b.free()
+    zk.lib.zkx_commit_trim(0)
+    return {"jobs": n, "plain_commits_per_s": round(n / plain_s, 2), "batched_commits_per_s": round(n / batch_s, 2),
+            "batched_over_plain": round(plain_s / batch_s, 3), "batched_equals_plain": bool(same),
+            "ms_per_batch": round(batch_s * 1e3, 2)}
+
+
+def config3(zk, np, steps, warmup, seed=3):
+    rng = np.random.default_rng(seed)
+    shapes = [(int(rng.choice([2, 20, 80, 135])), int(rng.integers(10, 17)), int(rng.integers(1, 4)), 0)
+              for _ in range(64)]
+    r = plain_vs_batched(zk, np, shapes, steps, warmup, seed)
+    r["workload"] = ("64 jobs: log2 degree U[10,16], columns from {2,20,80,135}, rate_bits {1,2,3}, "
+                     f"Poseidon, cap_height 4 (numpy seed {seed})")
+    return r
+
+
+def config4(zk, np, steps, warmup):
+    r = plain_vs_batched(zk, np, [(400, 20, 1, 3)] * 8, steps, warmup, 4)
+    r["workload"] = "8 jobs of 400 polys x 2^20, rate_bits 1 (2^21-point LDE), Poseidon2, cap_height 4, all resident"
+    return r
+
+
+def calibrate_copy(mib):
+    """A plain device copy of known size: three launches of an elementwise
+    int64 kernel reading `mib` MiB and writing `mib` MiB.  Under
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE its per-dispatch readings divided
+    by the known bytes give the counters' scale on this GPU."""
+    import torch
+    n = (mib << 20) // 8
+    x = torch.arange(n, dtype=torch.int64, device="cuda:0")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}))
 
 
 def perms_per_commit(rows, cols, caps):
@@ -72,7 +211,7 @@ def served_stream(zk, np, seconds, rate_hz, budget, streams, seed=5):
     rng = np.random.default_rng(seed)
     # one device pool of coefficients serves every request (its first cols*2^lg_n words)
     pool = zk.DeviceBuffer(400 << 20)
-    pool.zero()
+    pool.fill_random(seed)
     srv = zk.Server(budget, streams=streams, max_batch=64)
     arrivals, t = [], 0.0
     while True:
@@ -129,6 +268,9 @@ def main():
         print(f"bench.py: this build drives one GPU; --gpus {args.gpus} is not supported "
               "(jobs would shard across GPUs by commit, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
 
     import numpy as np
     import zeknox_amd as zk
@@ -136,7 +278,7 @@ def main():
 
     ngpu = zk.get_number_of_gpus_rs()
     assert ngpu >= 1, "no HIP device"
-    zk.init_coset_rs(0, LG, 7)
+    zk.init_coset_rs(0, max(LG, 21), 7)  # 2^21 points: the 400 x 2^20 batch
     zk.init_twiddle_factors_rs(0, LG)
 
     # synthetic inputs, resident in HBM
@@ -183,9 +325,11 @@ def main():
     commits = [zk.Commit(d_in, COLS, LG_N, RATE, CAP_H, HASH) for _ in range(args.jobs)]
     for _ in range(args.warmup):
         zk.commit_batch(commits)
+    zk.lib.zkx_device_synchronize(0)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        zk.commit_batch(commits)
+        zk.commit_batch(commits)  # returns when the batch is complete
+    zk.lib.zkx_device_synchronize(0)
     batch_step_s = (time.perf_counter() - t0) / args.steps
     batched_s = batch_step_s / args.jobs  # seconds per commit
     zk.kernel_stats_reset()
@@ -232,25 +376,16 @@ def main():
         ntt_roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(gbs / HBM_PEAK_GBS, 3), "traffic": None, "kernel": "batch_ntt (all passes)"}
 
-    # ---- CPU baseline: the C oracle on host cores, bounded sample
+    # ---- heterogeneous 64-job batch and the large Poseidon2 batch, both ways
+    extra = {}
+    if not args.no_extra:
+        extra["batch64"] = config3(zk, np, args.extra_steps, 1)
+        extra["large_poseidon2"] = config4(zk, np, args.extra_steps, 1)
+
+    # ---- CPU baseline: whole commits of the same input on the host cores
     cpu = None
     if not args.no_cpu:
-        import oracle_bind as ob
-        rows = args.cpu_rows
-        leaves = ob.canonical(rows * COLS, 7)
-        t = time.perf_counter()
-        ob.merkle(leaves, COLS, CAP_H, HASH)
-        merkle_s = time.perf_counter() - t
-        x = ob.canonical(COLS << LG_N, 9)
-        t = time.perf_counter()
-        lde = ob.lde(x, LG_N, RATE, COLS)
-        ob.transpose_rev(lde, LG, COLS)
-        lde_s = time.perf_counter() - t
-        commit_s = lde_s + merkle_s * (N / rows)
-        cpu = {"value": round(1.0 / commit_s, 4), "unit": "commits/s", "cores": ob.orc.orc_num_threads(),
-               "kind": "port",
-               "sample": f"full LDE+transpose of 135x2^15 -> 2^18 ({lde_s:.2f} s) + Poseidon Merkle over "
-                         f"{rows} of 2^18 leaf rows ({merkle_s:.2f} s), scaled to one commit"}
+        cpu = cpu_baseline(np, coeffs, args.cpu_runs, d_cap.to_host())
 
     served = None
     if args.serve_seconds > 0:
@@ -283,6 +418,7 @@ def main():
         "ntt_roofline": ntt_roofline,
         "cpu_baseline": cpu,
         "served": served,
+        **extra,
     }
     print(json.dumps(out))
 

@@ -27,6 +27,9 @@ RustError zkx_memcpy_htod(size_t device_id, void *dst, const void *src, size_t b
 RustError zkx_memcpy_dtoh(size_t device_id, void *dst, const void *src, size_t bytes);
 RustError zkx_memset(size_t device_id, void *dst, int value, size_t bytes);
 RustError zkx_device_synchronize(size_t device_id);
+/* Synthetic input on the device: dst[i] = splitmix64 of (seed, i), redrawn
+ * until below p, so uniform canonical field elements (benchmarks). */
+RustError zkx_fill_random(size_t device_id, void *dst, size_t count, uint64_t seed);
 
 /* The stream the library enqueues its work on (a hipStream_t). */
 void *zkx_stream(size_t device_id);
@@ -51,8 +54,11 @@ typedef struct {
 /* All jobs of one call run together (one launch per pass over all jobs);
  * returns when they are complete. */
 RustError zkx_commit_batch(size_t device_id, const ZkxCommitJob *jobs, size_t njobs);
-/* Device working memory zkx_commit_batch needs for these jobs, in bytes. */
+/* Device working memory zkx_commit_batch needs for these jobs, in bytes.
+ * The library keeps it between calls (one buffer per device, grown as
+ * needed); zkx_commit_trim frees it. */
 size_t zkx_commit_scratch_bytes(const ZkxCommitJob *jobs, size_t njobs);
+RustError zkx_commit_trim(size_t device_id);
 
 /* ---- commit server (continuous batching under an HBM budget) ------------
  * Requests are commit jobs as above whose output pointers are ignored: the

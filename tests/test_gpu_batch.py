@@ -105,3 +105,32 @@ def test_batch_rejects_bad_jobs(zk):
     bad.cap_height = 9
     with pytest.raises(zk.ZeknoxError):
         zk.commit_batch([bad])
+
+
+def test_batch_after_trim(zk):
+    """Freeing the engine's working memory between batches changes nothing."""
+    coeffs = ob.canonical(7 << 9, 71)
+    c = zk.Commit(coeffs, 7, 9, 2, 3, 3)
+    zk.commit_batch([c])
+    first = c.results()
+    assert zk.lib.zkx_commit_trim(0).code == 0
+    c2 = zk.Commit(coeffs, 7, 9, 2, 3, 3)
+    zk.commit_batch([c2])
+    for a, b in zip(first, c2.results()):
+        assert np.array_equal(a, b)
+    assert np.array_equal(first[2], oracle_commit(coeffs, 7, 9, 2, 3, 3)[2])
+
+
+def test_fill_random_is_canonical_and_seeded(zk):
+    n = 1 << 16
+    a, b, c = zk.DeviceBuffer(n), zk.DeviceBuffer(n), zk.DeviceBuffer(n)
+    a.fill_random(5)
+    b.fill_random(5)
+    c.fill_random(6)
+    x, y, z = a.to_host(), b.to_host(), c.to_host()
+    assert np.array_equal(x, y)
+    assert not np.array_equal(x, z)
+    assert int(x.max()) < ob.P and int(z.max()) < ob.P
+    assert len(np.unique(x)) == n
+    # uniform on [0, p): the mean of 2^16 draws is p/2 within 1%
+    assert abs(float(x.astype(np.float64).mean()) / ob.P - 0.5) < 0.01

@@ -125,6 +125,8 @@ def _load():
         "zkx_memcpy_htod": (RustError, [sz, vp, vp, sz]),
         "zkx_memcpy_dtoh": (RustError, [sz, vp, vp, sz]),
         "zkx_memset": (RustError, [sz, vp, ctypes.c_int, sz]),
+        "zkx_fill_random": (RustError, [sz, vp, sz, u64]),
+        "zkx_commit_trim": (RustError, [sz]),
         "zkx_device_synchronize": (RustError, [sz]),
         "zkx_stream": (vp, [sz]),
         "zkx_profile_enable": (RustError, [ctypes.c_int]),
@@ -194,6 +196,11 @@ class DeviceBuffer:
 
     def zero(self):
         _check(lib.zkx_memset(self.device_id, self.ptr, 0, self.count * 8))
+
+    def fill_random(self, seed, count=None):
+        """Uniform canonical field elements generated on the device."""
+        n = self.count if count is None else int(count)
+        _check(lib.zkx_fill_random(self.device_id, self.ptr, n, seed))
 
     def free(self):
         if self.ptr:

@@ -1,6 +1,7 @@
 """Build libzeknox.so (the HIP library for gfx950) in-tree with hipcc.
 
-    python -m zeknox_amd.build          # or zeknox_amd.build.build()
+    python3 zeknox_amd/build.py         # run as a script: importing the
+                                        # package would load the old library
 
 Every .hip file in zeknox_amd/csrc is compiled to an object under
 zeknox_amd/_build and linked into zeknox_amd/libzeknox.so.  The build always

@@ -52,6 +52,9 @@ public:
     void commit(const CommitJob *jobs, size_t n, void *scratch = nullptr);
     std::mutex mu;
     void sync();
+    // Wait for the engine's work, then free its working buffer (it grows
+    // back on the next commit that needs it).
+    void trim();
     hipStream_t stream() const { return stream_; }
     Device &device() { return dev_; }
 

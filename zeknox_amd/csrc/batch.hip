@@ -59,6 +59,13 @@ Engine::~Engine()
 
 void Engine::sync() { ZKX_HIP(hipStreamSynchronize(stream_)); }
 
+void Engine::trim()
+{
+    ZKX_HIP(hipSetDevice(dev_.id));
+    sync();
+    scratch_ = DevBuf();
+}
+
 void Engine::commit(const CommitJob *jobs, size_t n, void *scratch)
 {
     if (n == 0) return;

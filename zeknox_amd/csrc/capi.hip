@@ -316,6 +316,15 @@ RustError zkx_commit_batch(size_t device_id, const ZkxCommitJob *jobs, size_t nj
     });
 }
 
+RustError zkx_commit_trim(size_t device_id)
+{
+    return guarded([&] {
+        Engine &e = default_engine(device(device_id));
+        std::lock_guard<std::mutex> g(e.mu);
+        e.trim();
+    });
+}
+
 size_t zkx_commit_scratch_bytes(const ZkxCommitJob *jobs, size_t njobs)
 {
     try {

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/zeknox_ext.h"
+#include "ntt.h"
 #include "runtime.h"
 
 using namespace zkx;
@@ -139,6 +140,17 @@ RustError zkx_memset(size_t device_id, void *dst, int value, size_t bytes)
     return guarded([&] {
         Device &d = device(device_id);
         if (bytes) ZKX_HIP(hipMemsetAsync(dst, value, bytes, d.stream));
+        d.sync();
+    });
+}
+
+RustError zkx_fill_random(size_t device_id, void *dst, size_t count, uint64_t seed)
+{
+    return guarded([&] {
+        Device &d = device(device_id);
+        if (count && !dst) invalid("null pointer");
+        if (count > (0x7FFFFFFFull << 8)) invalid("fill too large for one launch");
+        fill_salt(static_cast<uint64_t *>(dst), count, seed, d.stream);
         d.sync();
     });
 }
