@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="served-stream duration (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=40.0, help="served-stream arrivals per second")
+    ap.add_argument("--no-tune", action="store_true", help="batched path on the default schedule (no autotuning)")
+    ap.add_argument("--schedule", default=None, metavar="S,E,T",
+                    help="batched config-2 schedule (ntt_strided_max,ntt_even,merkle_tile_log) instead of autotuning")
     ap.add_argument("--no-extra", action="store_true", help="skip the 64-job and 8 x 400 x 2^20 batches")
     ap.add_argument("--extra-steps", type=int, default=3, help="timed steps of those batches")
     ap.add_argument("--cpu-runs", type=int, default=5, help="whole CPU commits timed (median reported)")
@@ -117,7 +120,7 @@ def _timed(zk, fn, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
-def plain_vs_batched(zk, np, shapes, steps, warmup, seed):
+def plain_vs_batched(zk, np, shapes, steps, warmup, seed, tune):
     """Commit jobs of the given (cols, lg_n, rate, hash) shapes, cap height 4,
     coefficients generated on the device: the plain C-ABI sequence one job
     per call (shared LDE / leaf buffers) against one zkx_commit_batch of all
@@ -145,11 +148,17 @@ def plain_vs_batched(zk, np, shapes, steps, warmup, seed):
                                                        CAP_H, c.hash_type, 0)
 
     plain_s = _timed(zk, plain, steps, warmup)
+    tune_s = 0.0
+    if tune:  # one-time schedule search per distinct shape, outside the timing
+        t0 = time.perf_counter()
+        zk.autotune(commits)
+        tune_s = time.perf_counter() - t0
     batch_s = _timed(zk, lambda: zk.commit_batch(commits), steps, warmup)
     same = all(np.array_equal(cp.to_host(), c.d_caps.to_host())
                and np.array_equal(dg.to_host(c.num_digests * 4), c.d_digests.to_host(c.num_digests * 4))
                for c, dg, cp in zip(commits, p_dig, p_cap))
     n = len(commits)
+    schedules = sorted({zk.get_tuning(c).astuple() for c in commits})
     for b in [d_lde, d_leaves] + p_dig + p_cap:
         b.free()
     for c in commits:
@@ -159,21 +168,22 @@ def plain_vs_batched(zk, np, shapes, steps, warmup, seed):
     zk.lib.zkx_commit_trim(0)
     return {"jobs": n, "plain_commits_per_s": round(n / plain_s, 2), "batched_commits_per_s": round(n / batch_s, 2),
             "batched_over_plain": round(plain_s / batch_s, 3), "batched_equals_plain": bool(same),
-            "ms_per_batch": round(batch_s * 1e3, 2)}
+            "ms_per_batch": round(batch_s * 1e3, 2), "autotune_s": round(tune_s, 2),
+            "schedules (ntt_strided_max, ntt_even, merkle_tile_log)": [list(t) for t in schedules]}
 
 
-def config3(zk, np, steps, warmup, seed=3):
+def config3(zk, np, steps, warmup, tune, seed=3):
     rng = np.random.default_rng(seed)
     shapes = [(int(rng.choice([2, 20, 80, 135])), int(rng.integers(10, 17)), int(rng.integers(1, 4)), 0)
               for _ in range(64)]
-    r = plain_vs_batched(zk, np, shapes, steps, warmup, seed)
+    r = plain_vs_batched(zk, np, shapes, steps, warmup, seed, tune)
     r["workload"] = ("64 jobs: log2 degree U[10,16], columns from {2,20,80,135}, rate_bits {1,2,3}, "
                      f"Poseidon, cap_height 4 (numpy seed {seed})")
     return r
 
 
-def config4(zk, np, steps, warmup):
-    r = plain_vs_batched(zk, np, [(400, 20, 1, 3)] * 8, steps, warmup, 4)
+def config4(zk, np, steps, warmup, tune):
+    r = plain_vs_batched(zk, np, [(400, 20, 1, 3)] * 8, steps, warmup, 4, tune)
     r["workload"] = "8 jobs of 400 polys x 2^20, rate_bits 1 (2^21-point LDE), Poseidon2, cap_height 4, all resident"
     return r
 
@@ -323,6 +333,11 @@ def main():
 
     # ---- batched path: JOBS config-2 commits per zkx_commit_batch call
     commits = [zk.Commit(d_in, COLS, LG_N, RATE, CAP_H, HASH) for _ in range(args.jobs)]
+    if args.schedule:
+        zk.set_tuning(zk.Tuning(*[int(v) for v in args.schedule.split(",")]), commits[0])
+    elif not args.no_tune:  # one-time schedule search for the shape, outside the timing
+        zk.autotune(commits[:1])
+    schedule = dict(zip(("ntt_strided_max", "ntt_even", "merkle_tile_log"), zk.get_tuning(commits[0]).astuple()))
     for _ in range(args.warmup):
         zk.commit_batch(commits)
     zk.lib.zkx_device_synchronize(0)
@@ -379,8 +394,8 @@ def main():
     # ---- heterogeneous 64-job batch and the large Poseidon2 batch, both ways
     extra = {}
     if not args.no_extra:
-        extra["batch64"] = config3(zk, np, args.extra_steps, 1)
-        extra["large_poseidon2"] = config4(zk, np, args.extra_steps, 1)
+        extra["batch64"] = config3(zk, np, args.extra_steps, 1, not args.no_tune)
+        extra["large_poseidon2"] = config4(zk, np, args.extra_steps, 1, not args.no_tune)
 
     # ---- CPU baseline: whole commits of the same input on the host cores
     cpu = None
@@ -407,7 +422,8 @@ def main():
         "data": "synthetic (seeded uniform canonical field elements)",
         "config": {"workload": "LDE+commit: 135 polys x 2^15, rate_bits 3 (2^18-point coset LDE, shift 7), "
                                "transpose to 2^18 leaves x 135, Poseidon Merkle, cap_height 4",
-                   "path": mode, "commits_per_step": args.jobs, "parallelism": "single GPU"},
+                   "path": mode, "commits_per_step": args.jobs, "parallelism": "single GPU",
+                   "schedule": schedule},
         "plain_commits_per_s": round(1.0 / plain_s, 2),
         "batched_over_plain": round(plain_s / batched_s, 3),
         "batched_equals_plain": bit_exact,

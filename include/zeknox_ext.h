@@ -60,6 +60,40 @@ RustError zkx_commit_batch(size_t device_id, const ZkxCommitJob *jobs, size_t nj
 size_t zkx_commit_scratch_bytes(const ZkxCommitJob *jobs, size_t njobs);
 RustError zkx_commit_trim(size_t device_id);
 
+/* ---- schedule autotuning -------------------------------------------------
+ * How the commit of a shape (columns, log degree, rate bits, hash type, leaf
+ * rows kept) is cut into launches: the LDE's DIF stages per strided pass
+ * (1..9; greedy or spread evenly) and the leaves per Merkle work-group
+ * (2^8..2^10).  No setting changes a result.  Shapes without an entry use
+ * the default, which the C-ABI entry points (compute_batched_lde,
+ * fill_digests_buf_linear_gpu_with_gpu_ptr) use as well. */
+typedef struct {
+    uint32_t ntt_strided_max;  /* 1..9 */
+    uint32_t ntt_even;         /* 0 greedy, 1 even */
+    uint32_t merkle_tile_log;  /* 8..10 */
+    uint32_t reserved;
+} ZkxTuning;
+
+/* Times every distinct shape among the jobs under each schedule (`iters`
+ * runs each, fastest kept) and records the best for the shape.  The jobs'
+ * outputs are written with their results. */
+RustError zkx_autotune(size_t device_id, const ZkxCommitJob *jobs, size_t njobs, int iters);
+/* job NULL: the default. */
+RustError zkx_get_tuning(size_t device_id, const ZkxCommitJob *job, ZkxTuning *out);
+RustError zkx_set_tuning(size_t device_id, const ZkxCommitJob *job, const ZkxTuning *tuning);
+/* on: zkx_commit_batch and the server tune each shape they have not seen
+ * on its first commit (that commit takes longer). */
+RustError zkx_set_autotune(size_t device_id, int on);
+/* Stage counts of the DIF passes over 2^lg points under `tuning` (NULL: the
+ * built-in default), top stages first; returns the pass count (at most
+ * `cap` written), -1 for parameters out of range.  Host-only. */
+int zkx_ntt_pass_sizes(uint32_t lg, const ZkxTuning *tuning, uint32_t *sizes, int cap);
+/* Forget every shape entry and restore the built-in default. */
+RustError zkx_tuning_clear(size_t device_id);
+/* The shape table as text, one shape per line. */
+RustError zkx_tuning_save(size_t device_id, const char *path);
+RustError zkx_tuning_load(size_t device_id, const char *path);
+
 /* ---- commit server (continuous batching under an HBM budget) ------------
  * Requests are commit jobs as above whose output pointers are ignored: the
  * server allocates leaf rows (if want_leaves), digests and caps on the device

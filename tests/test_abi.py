@@ -74,3 +74,43 @@ def test_invalid_arguments_without_device():
     assert err.code == 1
     err = zk.lib.compute_batched_ntt(0, ctypes.addressof(out), 0, 0, zk.NTTConfig())
     assert err.code == 1
+
+
+# ---- NTT pass planner (host-only logic) --------------------------------------
+
+zk = pytest.importorskip("zeknox_amd")
+
+def _check_plan(lg, sizes, smax):
+    assert sum(sizes) == lg
+    assert sizes[-1] <= 12
+    below = lg
+    for m in sizes[:-1]:  # strided passes
+        below -= m
+        assert 1 <= m <= min(smax, 9)
+        assert below >= 12 - m  # stride at least the tile's column count
+
+
+@pytest.mark.parametrize("even", [0, 1])
+@pytest.mark.parametrize("smax", [1, 3, 4, 6, 8, 9])
+def test_ntt_pass_plans_are_valid(even, smax):
+    for lg in range(0, 33):
+        sizes = zk.ntt_pass_sizes(lg, zk.Tuning(smax, even, 8))
+        _check_plan(lg, sizes, smax)
+
+
+def test_ntt_pass_plan_defaults():
+    # greedy: strided passes of up to 9 stages, the last up to 12
+    assert zk.ntt_pass_sizes(10) == [10]
+    assert zk.ntt_pass_sizes(15) == [3, 12]
+    assert zk.ntt_pass_sizes(21) == [9, 12]
+    assert zk.ntt_pass_sizes(24) == [9, 3, 12]
+    # even: the same number of passes, stages spread
+    assert zk.ntt_pass_sizes(15, zk.Tuning(9, 1, 8)) == [8, 7]
+    assert zk.ntt_pass_sizes(24, zk.Tuning(9, 1, 8)) == [8, 8, 8]
+
+
+def test_tuning_out_of_range_rejected():
+    with pytest.raises(zk.ZeknoxError):
+        zk.ntt_pass_sizes(20, zk.Tuning(10, 0, 8))
+    with pytest.raises(zk.ZeknoxError):
+        zk.ntt_pass_sizes(20, zk.Tuning(9, 2, 8))

@@ -84,6 +84,18 @@ class CommitJob(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    """zeknox_ext.h ZkxTuning: schedule parameters of a commit shape."""
+    _fields_ = [("ntt_strided_max", ctypes.c_uint32), ("ntt_even", ctypes.c_uint32),
+                ("merkle_tile_log", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+    def __init__(self, ntt_strided_max=9, ntt_even=0, merkle_tile_log=8):
+        super().__init__(ntt_strided_max, ntt_even, merkle_tile_log, 0)
+
+    def astuple(self):
+        return (self.ntt_strided_max, self.ntt_even, self.merkle_tile_log)
+
+
 class ServeResult(ctypes.Structure):
     """zeknox_ext.h ZkxServeResult."""
     _fields_ = [("leaves", ctypes.c_void_p), ("digests", ctypes.c_void_p), ("caps", ctypes.c_void_p),
@@ -127,6 +139,14 @@ def _load():
         "zkx_memset": (RustError, [sz, vp, ctypes.c_int, sz]),
         "zkx_fill_random": (RustError, [sz, vp, sz, u64]),
         "zkx_commit_trim": (RustError, [sz]),
+        "zkx_autotune": (RustError, [sz, vp, sz, ctypes.c_int]),
+        "zkx_get_tuning": (RustError, [sz, vp, ctypes.POINTER(Tuning)]),
+        "zkx_set_tuning": (RustError, [sz, vp, ctypes.POINTER(Tuning)]),
+        "zkx_set_autotune": (RustError, [sz, ctypes.c_int]),
+        "zkx_tuning_clear": (RustError, [sz]),
+        "zkx_tuning_save": (RustError, [sz, ctypes.c_char_p]),
+        "zkx_tuning_load": (RustError, [sz, ctypes.c_char_p]),
+        "zkx_ntt_pass_sizes": (ctypes.c_int, [u32, ctypes.POINTER(Tuning), ctypes.POINTER(u32), ctypes.c_int]),
         "zkx_device_synchronize": (RustError, [sz]),
         "zkx_stream": (vp, [sz]),
         "zkx_profile_enable": (RustError, [ctypes.c_int]),
@@ -342,6 +362,53 @@ def commit_batch(commits, device_id=0):
 def commit_scratch_bytes(commits):
     arr = (CommitJob * len(commits))(*[c.job() for c in commits])
     return lib.zkx_commit_scratch_bytes(arr, len(commits))
+
+
+# ---- schedule autotuning ---------------------------------------------------------
+
+def autotune(commits, iters=3, device_id=0):
+    """Tune every distinct shape among the Commit objects (their outputs are
+    written with their results)."""
+    arr = (CommitJob * len(commits))(*[c.job() for c in commits])
+    _check(lib.zkx_autotune(device_id, arr, len(commits), iters))
+
+
+def get_tuning(commit=None, device_id=0):
+    t = Tuning()
+    job = ctypes.byref(commit.job()) if commit is not None else None
+    _check(lib.zkx_get_tuning(device_id, job, ctypes.byref(t)))
+    return t
+
+
+def set_tuning(tuning, commit=None, device_id=0):
+    """Schedule of the commit's shape, or the default (commit None)."""
+    job = ctypes.byref(commit.job()) if commit is not None else None
+    _check(lib.zkx_set_tuning(device_id, job, ctypes.byref(tuning)))
+
+
+def set_autotune(on, device_id=0):
+    _check(lib.zkx_set_autotune(device_id, int(bool(on))))
+
+
+def tuning_clear(device_id=0):
+    _check(lib.zkx_tuning_clear(device_id))
+
+
+def tuning_save(path, device_id=0):
+    _check(lib.zkx_tuning_save(device_id, os.fsencode(path)))
+
+
+def tuning_load(path, device_id=0):
+    _check(lib.zkx_tuning_load(device_id, os.fsencode(path)))
+
+
+def ntt_pass_sizes(lg, tuning=None):
+    """Stage counts of the DIF passes over 2^lg points (host-only)."""
+    out = (ctypes.c_uint32 * 40)()
+    n = lib.zkx_ntt_pass_sizes(lg, ctypes.byref(tuning) if tuning is not None else None, out, 40)
+    if n < 0:
+        raise ZeknoxError(1, "schedule parameters out of range")
+    return list(out[:n])
 
 
 # ---- commit server -----------------------------------------------------------

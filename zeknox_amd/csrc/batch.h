@@ -21,6 +21,7 @@
 #include "merkle.h"
 #include "ntt.h"
 #include "runtime.h"
+#include "tuning.h"
 
 namespace zkx {
 
@@ -49,7 +50,8 @@ public:
     // serialised by `mu`.  `scratch` (>= commit_scratch_bytes, device,
     // stream-ordered on this engine's stream) replaces the engine's own
     // working buffer when given.
-    void commit(const CommitJob *jobs, size_t n, void *scratch = nullptr);
+    // `forced`, when given, replaces every job's schedule parameters.
+    void commit(const CommitJob *jobs, size_t n, void *scratch = nullptr, const Tuning *forced = nullptr);
     std::mutex mu;
     void sync();
     // Wait for the engine's work, then free its working buffer (it grows
@@ -73,5 +75,13 @@ private:
 
 // The device's engine on the device stream (used by zkx_commit_batch).
 Engine &default_engine(Device &d);
+
+// Shape key of a job for the device's tuning table.
+TuneKey tune_key(const CommitJob &j);
+// Times the job on `e` under each distinct schedule (NTT pass split, then
+// Merkle work-group size; `iters` runs each, fastest kept), records the best
+// for the job's shape on the device and returns it.  The job's outputs are
+// written (every schedule computes the same result).  Call with e.mu held.
+Tuning autotune(Engine &e, const CommitJob &job, int iters, void *scratch = nullptr);
 
 }  // namespace zkx

@@ -66,11 +66,25 @@ void Engine::trim()
     scratch_ = DevBuf();
 }
 
-void Engine::commit(const CommitJob *jobs, size_t n, void *scratch)
+void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning *forced)
 {
     if (n == 0) return;
     for (size_t i = 0; i < n; i++) check_commit_job(jobs[i]);
+    if (forced && !valid_tuning(*forced)) invalid("schedule parameters out of range");
     ZKX_HIP(hipSetDevice(dev_.id));
+
+    // ---- schedule parameters of each job's shape: forced, the device's
+    // entry for the shape, or (autotuning on, shape unseen) tuned now
+    std::vector<Tuning> tun(n);
+    for (size_t i = 0; i < n; i++) {
+        if (forced) {
+            tun[i] = *forced;
+            continue;
+        }
+        bool found = false;
+        tun[i] = dev_.tuning_for(tune_key(jobs[i]), &found);
+        if (!found && dev_.autotune_new_shapes) tun[i] = autotune(*this, jobs[i], 3, scratch);
+    }
 
     // ---- working memory: every job's LDE in DIF (bit-reversed) order
     const size_t need = commit_scratch_bytes(jobs, n);
@@ -117,7 +131,7 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch)
         const CommitJob &j = jobs[i];
         const uint32_t lg = j.lg_n + j.rate;
         ntt_tiles[i] = plan_dif(ntt_plan[i], lde[i], (uint64_t)j.cols << j.rate, j.lg_n, lg, tw[i], tw12, false,
-                                kSrcLde, j.coeffs, j.rate, false, F[i]);
+                                kSrcLde, j.coeffs, j.rate, false, F[i], tun[i]);
         MerkleSpec s;
         s.digests = j.digests;
         s.caps = j.caps;
@@ -132,7 +146,7 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch)
         s.leaf_stride = 1;
         s.col_stride = 1ull << lg;
         check_merkle(s);
-        plan_merkle(s, mk_plan[i]);
+        plan_merkle(s, mk_plan[i], tun[i].merkle_tile_log);
         max_ntt = std::max(max_ntt, ntt_plan[i].size());
         max_mk = std::max(max_mk, mk_plan[i].size());
     }
@@ -144,8 +158,9 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch)
         uint32_t count;
         uint64_t blocks;
         int kind;       // 0 NTT, 1 Merkle
-        int hash;
+        int hash;       // NTT: contiguous; Merkle: hash type
         bool leaves;
+        uint32_t tile_log;  // Merkle work-group size (log2 lanes)
     };
     std::vector<Launch> launches;
     std::vector<char> host;
@@ -168,25 +183,27 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch)
             }
             if (list.empty()) continue;
             const size_t off = push(list.data(), list.size() * sizeof(NttPass));
-            launches.push_back({off, (uint32_t)list.size(), blocks, 0, contig, false});
+            launches.push_back({off, (uint32_t)list.size(), blocks, 0, contig, false, 0});
             host.resize(align_up(host.size()));
         }
     for (size_t k = 0; k < max_mk; k++)
-        for (int ht : {0, 3}) {
-            std::vector<MerkleArgs> list;
-            uint64_t blocks = 0;
-            for (size_t i = 0; i < n; i++) {
-                if ((int)jobs[i].hash_type != ht || k >= mk_plan[i].size()) continue;
-                MerkleArgs a = mk_plan[i][k];
-                a.tile_begin = blocks;
-                blocks += a.count >> a.tile_log;
-                list.push_back(a);
+        for (int ht : {0, 3})
+            for (uint32_t tl = kMerkleMinTileLog; tl <= (uint32_t)kMerkleMaxTileLog; tl++) {
+                std::vector<MerkleArgs> list;
+                uint64_t blocks = 0;
+                for (size_t i = 0; i < n; i++) {
+                    if ((int)jobs[i].hash_type != ht || k >= mk_plan[i].size() || tun[i].merkle_tile_log != tl)
+                        continue;
+                    MerkleArgs a = mk_plan[i][k];
+                    a.tile_begin = blocks;
+                    blocks += a.count >> a.tile_log;
+                    list.push_back(a);
+                }
+                if (list.empty()) continue;
+                const size_t off = push(list.data(), list.size() * sizeof(MerkleArgs));
+                launches.push_back({off, (uint32_t)list.size(), blocks, 1, ht, k == 0, tl});
+                host.resize(align_up(host.size()));
             }
-            if (list.empty()) continue;
-            const size_t off = push(list.data(), list.size() * sizeof(MerkleArgs));
-            launches.push_back({off, (uint32_t)list.size(), blocks, 1, ht, k == 0});
-            host.resize(align_up(host.size()));
-        }
     size_t off0 = 0;
     stage(host.data(), host.size(), off0);
     (void)off0;
@@ -199,7 +216,7 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch)
                             "batch_ntt");
         } else {
             launch_merkle_pass(stream_, l.hash, l.leaves, reinterpret_cast<const MerkleArgs *>(base), l.count,
-                               MerkleArgs{}, l.blocks);
+                               MerkleArgs{}, l.blocks, l.tile_log);
         }
     }
 }

@@ -6,6 +6,7 @@
 #include <string.h>
 #include <time.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -322,6 +323,145 @@ RustError zkx_commit_trim(size_t device_id)
         Engine &e = default_engine(device(device_id));
         std::lock_guard<std::mutex> g(e.mu);
         e.trim();
+    });
+}
+
+RustError zkx_autotune(size_t device_id, const ZkxCommitJob *jobs, size_t njobs, int iters)
+{
+    return guarded([&] {
+        std::vector<CommitJob> v = to_jobs(jobs, njobs);
+        Device &d = device(device_id);
+        Engine &e = default_engine(d);
+        std::lock_guard<std::mutex> g(e.mu);
+        std::vector<TuneKey> done;
+        for (const CommitJob &j : v) {
+            const TuneKey k = tune_key(j);
+            if (std::find(done.begin(), done.end(), k) != done.end()) continue;
+            autotune(e, j, iters);
+            done.push_back(k);
+        }
+        e.sync();
+    });
+}
+
+RustError zkx_get_tuning(size_t device_id, const ZkxCommitJob *job, ZkxTuning *out)
+{
+    return guarded([&] {
+        if (!out) invalid("null output");
+        Device &d = device(device_id);
+        Tuning t = d.tuning_default();
+        if (job) t = d.tuning_for(tune_key(to_jobs(job, 1)[0]));
+        *out = ZkxTuning{t.ntt_strided_max, t.ntt_even, t.merkle_tile_log, 0};
+    });
+}
+
+RustError zkx_set_tuning(size_t device_id, const ZkxCommitJob *job, const ZkxTuning *t)
+{
+    return guarded([&] {
+        if (!t) invalid("null tuning");
+        Tuning v;
+        v.ntt_strided_max = t->ntt_strided_max;
+        v.ntt_even = t->ntt_even;
+        v.merkle_tile_log = t->merkle_tile_log;
+        if (!valid_tuning(v)) invalid("schedule parameters out of range");
+        Device &d = device(device_id);
+        std::lock_guard<std::mutex> g(d.tune_mu);
+        if (job) d.tuned[tune_key(to_jobs(job, 1)[0])] = v;
+        else d.tune_default = v;
+    });
+}
+
+int zkx_ntt_pass_sizes(uint32_t lg, const ZkxTuning *t, uint32_t *sizes, int cap)
+{
+    try {
+        Tuning v;
+        if (t) {
+            v.ntt_strided_max = t->ntt_strided_max;
+            v.ntt_even = t->ntt_even;
+            v.merkle_tile_log = t->merkle_tile_log;
+        }
+        if (!valid_tuning(v) || lg > 32) return -1;
+        const std::vector<uint32_t> p = ntt_pass_sizes(lg, v);
+        for (size_t i = 0; i < p.size() && (int)i < cap; i++) sizes[i] = p[i];
+        return (int)p.size();
+    } catch (...) {
+        return -1;
+    }
+}
+
+RustError zkx_set_autotune(size_t device_id, int on)
+{
+    return guarded([&] {
+        Device &d = device(device_id);
+        std::lock_guard<std::mutex> g(d.tune_mu);
+        d.autotune_new_shapes = on != 0;
+    });
+}
+
+RustError zkx_tuning_clear(size_t device_id)
+{
+    return guarded([&] {
+        Device &d = device(device_id);
+        std::lock_guard<std::mutex> g(d.tune_mu);
+        d.tuned.clear();
+        d.tune_default = Tuning{};
+    });
+}
+
+RustError zkx_tuning_save(size_t device_id, const char *path)
+{
+    return guarded([&] {
+        if (!path) invalid("null path");
+        Device &d = device(device_id);
+        std::lock_guard<std::mutex> g(d.tune_mu);
+        FILE *f = fopen(path, "w");
+        if (!f) invalid(std::string("cannot open ") + path);
+        fprintf(f, "# cols lg_n rate hash leaves : ntt_strided_max ntt_even merkle_tile_log\n");
+        for (const auto &kv : d.tuned) {
+            const TuneKey &k = kv.first;
+            const Tuning &t = kv.second;
+            fprintf(f, "%u %u %u %u %d : %u %u %u\n", std::get<0>(k), std::get<1>(k), std::get<2>(k),
+                    std::get<3>(k), (int)std::get<4>(k), t.ntt_strided_max, t.ntt_even, t.merkle_tile_log);
+        }
+        const bool ok = fclose(f) == 0;
+        if (!ok) invalid(std::string("cannot write ") + path);
+    });
+}
+
+RustError zkx_tuning_load(size_t device_id, const char *path)
+{
+    return guarded([&] {
+        if (!path) invalid("null path");
+        Device &d = device(device_id);
+        FILE *f = fopen(path, "r");
+        if (!f) invalid(std::string("cannot open ") + path);
+        std::vector<std::pair<TuneKey, Tuning>> rows;
+        char line[256];
+        int lineno = 0;
+        bool bad = false;
+        while (fgets(line, sizeof line, f)) {
+            lineno++;
+            if (line[0] == '#' || line[0] == '\n') continue;
+            unsigned c, n, r, h, s, e, t;
+            int lv;
+            if (sscanf(line, "%u %u %u %u %d : %u %u %u", &c, &n, &r, &h, &lv, &s, &e, &t) != 8) {
+                bad = true;
+                break;
+            }
+            Tuning v;
+            v.ntt_strided_max = s;
+            v.ntt_even = e;
+            v.merkle_tile_log = t;
+            if (!valid_tuning(v)) {
+                bad = true;
+                break;
+            }
+            rows.push_back({TuneKey{c, n, r, h, lv != 0}, v});
+        }
+        fclose(f);
+        if (bad) invalid(std::string(path) + ": bad line " + std::to_string(lineno));
+        std::lock_guard<std::mutex> g(d.tune_mu);
+        for (auto &kv : rows) d.tuned[kv.first] = kv.second;
     });
 }
 

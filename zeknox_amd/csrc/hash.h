@@ -26,19 +26,21 @@ static __constant__ uint64_t c_poseidon2_rc[30 * 12] = {ZK_POSEIDON2_RC_LIST};
 
 // ---------------------------------------------------------------- Poseidon
 
-// value = A + B * 2^32 (A, B < 2^64 - 2^32, value < 2^96), reduced: the
-// middle word is B + hi(A) formed with one v_mad_u64_u32 (x * 1 + y).
+// value = A + B * 2^32 (A, B < 2^64 - 2^32, value < 2^96), reduced lazily
+// (below 2^64, congruent mod p): the middle word is B + hi(A) formed with
+// one v_mad_u64_u32 (x * 1 + y).
 __device__ __forceinline__ uint64_t limb_combine(uint64_t A, uint64_t B)
 {
     const uint64_t T = gl::add_small(B, gl::hi32(A));  // < 2^64
-    return gl::reduce96(gl::pack(gl::lo32(A), gl::lo32(T)), gl::hi32(T));
+    return gl::reduce96_lazy(gl::pack(gl::lo32(A), gl::lo32(T)), gl::hi32(T));
 }
 
 // state <- MDS * state + k (k: 12 constants, the next round's), with
 // MDS = circ(17,15,41,16,2,28,13,13,39,18,34,20) + diag(8,0,...,0).
 // Each output row is accumulated in 32-bit limbs, A = k_lo + sum c*lo(x),
 // B = k_hi + sum c*hi(x), one v_mad_u64_u32 per product (coefficients are
-// inline constants; row coefficient sums <= 264 keep A, B < 2^41).
+// inline constants; row coefficient sums <= 264 keep A, B < 2^41 for any
+// x < 2^64).  Inputs and outputs are lazy (below 2^64, congruent mod p).
 __device__ __forceinline__ void poseidon_mds(uint64_t s[12], const uint64_t *k)
 {
     uint32_t lo[12], hi[12];
@@ -58,13 +60,17 @@ __device__ __forceinline__ void poseidon_mds(uint64_t s[12], const uint64_t *k)
             // the first product takes the SGPR constant pair as its addend, so
             // its coefficient stays an inline constant (one SGPR per VALU op)
             const uint32_t c = i == 0 ? c0 : gl::opaque(c0);
-            A = (uint64_t)lo[j] * c + A;
-            B = (uint64_t)hi[j] * c + B;
+            // each step pinned: one v_mad_u64_u32 per product, no regrouping
+            A = gl::vopaque64((uint64_t)lo[j] * c + A);
+            B = gl::vopaque64((uint64_t)hi[j] * c + B);
         }
         s[r] = limb_combine(A, B);
     }
 }
 
+// Lazy in and out: the state stays below 2^64 and congruent mod p; the
+// sponge canonicalises what it outputs.  (gl::add(a, b) with a < 2^64 and
+// b < p returns a value below 2^64 congruent to a + b.)
 __device__ __forceinline__ void poseidon_permute(uint64_t s[12])
 {
 #pragma unroll
@@ -72,10 +78,10 @@ __device__ __forceinline__ void poseidon_permute(uint64_t s[12])
     // rounds 0-3 and 26-29 are full (S-box on all 12), 4-25 partial (s0 only)
 #pragma unroll 1
     for (int r = 0; r < 30; r++) {
-        s[0] = gl::pow7(s[0]);
+        s[0] = gl::pow7_lazy(s[0]);
         if (r < 4 || r >= 26) {
 #pragma unroll
-            for (int i = 1; i < 12; i++) s[i] = gl::pow7(s[i]);
+            for (int i = 1; i < 12; i++) s[i] = gl::pow7_lazy(s[i]);
         }
         poseidon_mds(s, c_poseidon_rc + 12 * (r + 1));
     }
@@ -83,7 +89,7 @@ __device__ __forceinline__ void poseidon_permute(uint64_t s[12])
 
 // --------------------------------------------------------------- Poseidon2
 
-// External layer M_E = [[2M4, M4, M4], [M4, 2M4, M4], [M4, M4, 2M4]] with
+// External layer (lazy in and out) M_E = [[2M4, M4, M4], [M4, 2M4, M4], [M4, M4, 2M4]] with
 // M4 = [[5,7,1,3],[4,6,1,1],[1,3,5,7],[1,1,4,6]] (poseidon2.cu:65-145: M4 on
 // each 4-block, then each position plus the sum of that position over the
 // blocks).  out_b = M4 * (s_b + u), u[c] = s[c] + s[4+c] + s[8+c]; the shared
@@ -105,8 +111,8 @@ __device__ __forceinline__ void p2_external(uint64_t s[12])
         for (int c = 0; c < 4; c++)
 #pragma unroll
             for (int b = 0; b < 12; b += 4) {
-                A = (uint64_t)lo[b + c] * gl::opaque(M4[r][c]) + A;
-                B = (uint64_t)hi[b + c] * gl::opaque(M4[r][c]) + B;
+                A = gl::vopaque64((uint64_t)lo[b + c] * gl::opaque(M4[r][c]) + A);
+                B = gl::vopaque64((uint64_t)hi[b + c] * gl::opaque(M4[r][c]) + B);
             }
         VA[r] = A;
         VB[r] = B;
@@ -118,15 +124,15 @@ __device__ __forceinline__ void p2_external(uint64_t s[12])
             uint64_t A = VA[r], B = VB[r];
 #pragma unroll
             for (int c = 0; c < 4; c++) {
-                A = (uint64_t)lo[b + c] * gl::opaque(M4[r][c]) + A;
-                B = (uint64_t)hi[b + c] * gl::opaque(M4[r][c]) + B;
+                A = gl::vopaque64((uint64_t)lo[b + c] * gl::opaque(M4[r][c]) + A);
+                B = gl::vopaque64((uint64_t)hi[b + c] * gl::opaque(M4[r][c]) + B);
             }
             s[b + r] = limb_combine(A, B);
         }
 }
 
-// Canonical sum of the 12 state elements: 32-bit carry chain into three
-// words, one reduction.
+// Canonical sum of the 12 state elements (each below 2^64): 32-bit carry
+// chain into three words, one reduction.
 __device__ __forceinline__ uint64_t sum12(const uint64_t s[12])
 {
     uint32_t lo = 0, hi = 0, top = 0;
@@ -141,7 +147,7 @@ __device__ __forceinline__ uint64_t sum12(const uint64_t s[12])
 }
 
 // Internal layer: s_i <- s_i * d_i + sum(s) (d = MATRIX_DIAG_12 = diag - 1),
-// each as one 128-bit value reduced once.
+// each as one 128-bit value reduced once (lazy inputs, canonical outputs).
 __device__ __forceinline__ void p2_internal(uint64_t s[12])
 {
     const uint64_t sum = sum12(s);
@@ -158,6 +164,7 @@ __device__ __forceinline__ void p2_internal(uint64_t s[12])
     }
 }
 
+// Lazy in and out, like poseidon_permute.
 __device__ __forceinline__ void poseidon2_permute(uint64_t s[12])
 {
     p2_external(s);
@@ -165,10 +172,10 @@ __device__ __forceinline__ void poseidon2_permute(uint64_t s[12])
 #pragma unroll 1
     for (int r = 0; r < 30; r++) {
         const uint64_t *rc = c_poseidon2_rc + 12 * r;
-        s[0] = gl::pow7(gl::add(s[0], rc[0]));
+        s[0] = gl::pow7_lazy(gl::add(s[0], rc[0]));
         if (r < 4 || r >= 26) {
 #pragma unroll
-            for (int i = 1; i < 12; i++) s[i] = gl::pow7(gl::add(s[i], rc[i]));
+            for (int i = 1; i < 12; i++) s[i] = gl::pow7_lazy(gl::add(s[i], rc[i]));
             p2_external(s);
         } else {
             p2_internal(s);
@@ -196,7 +203,7 @@ __device__ __forceinline__ void hash_two(const uint64_t l[4], const uint64_t r[4
     }
     permute<HT>(s);
 #pragma unroll
-    for (int i = 0; i < 4; i++) out[i] = s[i];
+    for (int i = 0; i < 4; i++) out[i] = gl::canon(s[i]);
 }
 
 // hash_one over a leaf of n elements fetched through load(k):
@@ -220,7 +227,7 @@ __device__ __forceinline__ void hash_one(uint32_t n, Load load, uint64_t out[4])
         permute<HT>(s);
     }
 #pragma unroll
-    for (int i = 0; i < 4; i++) out[i] = s[i];
+    for (int i = 0; i < 4; i++) out[i] = gl::canon(s[i]);
 }
 
 }  // namespace zkx

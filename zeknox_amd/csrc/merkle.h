@@ -20,12 +20,14 @@ struct MerkleSpec {
 
 // Throws Error(1, ...) when the sizes are not a linear Merkle forest.
 void check_merkle(const MerkleSpec &s);
-// Pass descriptors of one tree, in launch order (tile_begin left 0).
-void plan_merkle(const MerkleSpec &s, std::vector<MerkleArgs> &passes);
+// Pass descriptors of one tree, in launch order (tile_begin left 0), with
+// tiles of at most 2^tile_log_max nodes (8..10).
+void plan_merkle(const MerkleSpec &s, std::vector<MerkleArgs> &passes, uint32_t tile_log_max);
 // One launch over `n` job descriptors in device memory (n > 0), or over the
-// single descriptor `one` (n == 0); `blocks` work-groups in total.
+// single descriptor `one` (n == 0); `blocks` work-groups of 2^tile_log_max
+// lanes in total (every descriptor's tile_log <= tile_log_max).
 void launch_merkle_pass(hipStream_t stream, int hash_type, bool from_leaves, const MerkleArgs *dev_list,
-                        uint32_t n, const MerkleArgs &one, uint64_t blocks);
+                        uint32_t n, const MerkleArgs &one, uint64_t blocks, uint32_t tile_log_max);
 void merkle_build(Device &d, const MerkleSpec &s);
 
 }  // namespace zkx

@@ -32,8 +32,10 @@ void check_merkle(const MerkleSpec &s)
     if (!s.digests) invalid("null digest buffer");
 }
 
-void plan_merkle(const MerkleSpec &s, std::vector<MerkleArgs> &passes)
+void plan_merkle(const MerkleSpec &s, std::vector<MerkleArgs> &passes, uint32_t tile_log_max)
 {
+    if (tile_log_max < (uint32_t)kMerkleMinTileLog || tile_log_max > (uint32_t)kMerkleMaxTileLog)
+        invalid("Merkle tile size out of range");
     MerkleArgs a{};
     a.leaves = s.leaves;
     a.leaves_out = s.leaves_out;
@@ -45,7 +47,7 @@ void plan_merkle(const MerkleSpec &s, std::vector<MerkleArgs> &passes)
     if (s.num_caps == s.num_leaves) {
         a.count = s.num_leaves;
         a.lg_w = 0;
-        a.tile_log = std::min<uint32_t>(8, log2u(a.count));
+        a.tile_log = std::min<uint32_t>(tile_log_max, log2u(a.count));
         passes.push_back(a);
         return;
     }
@@ -55,39 +57,58 @@ void plan_merkle(const MerkleSpec &s, std::vector<MerkleArgs> &passes)
     while (true) {
         a.count = s.num_caps << lg_w;
         a.lg_w = lg_w;
-        a.tile_log = std::min<uint32_t>(8, log2u(a.count));
+        a.tile_log = std::min<uint32_t>(tile_log_max, log2u(a.count));
         passes.push_back(a);
         if (lg_w <= a.tile_log) break;  // this pass reaches the caps
         lg_w -= a.tile_log;
     }
 }
 
+namespace {
+
+template <int HT, int TPB>
+void launch_tpb(hipStream_t stream, bool from_leaves, const MerkleArgs *dev_list, uint32_t n, const MerkleArgs &one,
+                uint64_t blocks)
+{
+    const dim3 g((uint32_t)blocks), b(TPB);
+    if (from_leaves) hipLaunchKernelGGL((k_merkle_tile<HT, true, TPB>), g, b, 0, stream, dev_list, n, one);
+    else hipLaunchKernelGGL((k_merkle_tile<HT, false, TPB>), g, b, 0, stream, dev_list, n, one);
+}
+
+template <int HT>
+void launch_hash(hipStream_t stream, bool from_leaves, const MerkleArgs *dev_list, uint32_t n, const MerkleArgs &one,
+                 uint64_t blocks, uint32_t tile_log_max)
+{
+    if (tile_log_max == 10) launch_tpb<HT, 1024>(stream, from_leaves, dev_list, n, one, blocks);
+    else if (tile_log_max == 9) launch_tpb<HT, 512>(stream, from_leaves, dev_list, n, one, blocks);
+    else launch_tpb<HT, 256>(stream, from_leaves, dev_list, n, one, blocks);
+}
+
+}  // namespace
+
 void launch_merkle_pass(hipStream_t stream, int hash_type, bool from_leaves, const MerkleArgs *dev_list,
-                        uint32_t n, const MerkleArgs &one, uint64_t blocks)
+                        uint32_t n, const MerkleArgs &one, uint64_t blocks, uint32_t tile_log_max)
 {
     if (blocks == 0) return;
     if (blocks > 0x7FFFFFFFull) invalid("tree too large for one launch");
+    if (tile_log_max < (uint32_t)kMerkleMinTileLog || tile_log_max > (uint32_t)kMerkleMaxTileLog)
+        invalid("Merkle tile size out of range");
     ProfScope ps(from_leaves ? "merkle_leaves" : "merkle_levels", stream);
-    const dim3 g((uint32_t)blocks), b(kMerkleThreads);
-    if (hash_type == kPoseidon2) {
-        if (from_leaves) hipLaunchKernelGGL((k_merkle_tile<kPoseidon2, true>), g, b, 0, stream, dev_list, n, one);
-        else hipLaunchKernelGGL((k_merkle_tile<kPoseidon2, false>), g, b, 0, stream, dev_list, n, one);
-    } else {
-        if (from_leaves) hipLaunchKernelGGL((k_merkle_tile<kPoseidon, true>), g, b, 0, stream, dev_list, n, one);
-        else hipLaunchKernelGGL((k_merkle_tile<kPoseidon, false>), g, b, 0, stream, dev_list, n, one);
-    }
+    if (hash_type == kPoseidon2) launch_hash<kPoseidon2>(stream, from_leaves, dev_list, n, one, blocks, tile_log_max);
+    else launch_hash<kPoseidon>(stream, from_leaves, dev_list, n, one, blocks, tile_log_max);
     ZKX_HIP(hipGetLastError());
 }
 
 void merkle_build(Device &d, const MerkleSpec &s)
 {
     check_merkle(s);
+    const uint32_t tl = d.tuning_default().merkle_tile_log;
     std::vector<MerkleArgs> passes;
-    plan_merkle(s, passes);
+    plan_merkle(s, passes, tl);
     bool first = true;
     for (MerkleArgs &p : passes) {
         p.tile_begin = 0;
-        launch_merkle_pass(d.stream, (int)s.hash_type, first, nullptr, 0, p, p.count >> p.tile_log);
+        launch_merkle_pass(d.stream, (int)s.hash_type, first, nullptr, 0, p, p.count >> p.tile_log, tl);
         first = false;
     }
 }

@@ -7,14 +7,14 @@
 // so node i has children 2i+2 and 2i+3, the level of width w sits at
 // [w-2, 2w-2), leaf hashes at [sl-2, 2sl-2), cap[s] = H(d_s[0], d_s[1]).
 //
-// Schedule.  A work-group of up to 256 lanes owns a tile of consecutive
+// Schedule.  A work-group of TPB (256..1024) lanes owns a tile of consecutive
 // nodes of one level (leaves in the first pass).  Each lane hashes its leaf
 // (or loads its node), then the tile is reduced level by level in LDS: every
 // level's digests are written to HBM on the way up, and a tile that reaches
 // the width-2 top level of its subtree also writes the cap.  A tile that
 // stops below the top leaves its root for the next pass, which starts from
 // that level; passes repeat until the caps are written (one pass when a
-// subtree has <= 256 leaves, two up to 2^16, three up to 2^24).
+// subtree has <= TPB leaves, two up to TPB^2, ...).
 //
 // Leaf sources.  Element k of leaf t is read at t*leaf_stride + k*col_stride:
 // row-major leaves (the C ABI) use (leaf_size, 1); the commit engine hashes
@@ -48,11 +48,11 @@ __device__ __forceinline__ const MerkleArgs *find_job(const MerkleArgs *jobs, ui
     return jobs + lo;
 }
 
-template <int HT, bool FROM_LEAVES>
-__global__ void __launch_bounds__(kMerkleThreads) k_merkle_tile(const MerkleArgs *__restrict__ jobs, uint32_t njobs,
-                                                              MerkleArgs one)
+template <int HT, bool FROM_LEAVES, int TPB>
+__global__ void __launch_bounds__(TPB) k_merkle_tile(const MerkleArgs *__restrict__ jobs, uint32_t njobs,
+                                                   MerkleArgs one)
 {
-    __shared__ uint64_t lds[kMerkleThreads][4];
+    __shared__ uint64_t lds[TPB][4];
     const MerkleArgs a = njobs ? *find_job(jobs, njobs) : one;
     const uint32_t t = threadIdx.x;
     uint32_t cur = 1u << a.tile_log;  // nodes of this tile at the current level

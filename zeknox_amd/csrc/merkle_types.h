@@ -5,7 +5,10 @@
 
 namespace zkx {
 
-constexpr int kMerkleThreads = 256;
+// Work-group sizes of the Merkle kernels: 2^8, 2^9 or 2^10 lanes, one leaf
+// (or node) per lane.
+constexpr int kMerkleMinTileLog = 8;
+constexpr int kMerkleMaxTileLog = 10;
 
 struct MerkleArgs {
     const uint64_t *leaves;  // first pass: leaf source

@@ -5,6 +5,7 @@
 
 #include "ntt_types.h"
 #include "runtime.h"
+#include "tuning.h"
 
 namespace zkx {
 
@@ -27,11 +28,15 @@ void bitrev_columns(Device &d, uint64_t *buf, uint32_t lg, uint32_t cols, bool u
 void transpose_rev(Device &d, uint64_t *out, const uint64_t *in, uint32_t lg, uint32_t cols, bool rev);
 
 // Pass descriptors of the DIF over nblk blocks of 2^L in buf (in launch
-// order; each pass must finish before the next starts).  Returns the tile
-// count of every pass.
+// order; each pass must finish before the next starts), cut into passes as
+// `tun` says.  Returns the tile count of every pass.
 uint64_t plan_dif(std::vector<NttPass> &out, uint64_t *buf, uint64_t nblk, uint32_t L, uint32_t lg_tw,
                   const uint64_t *tw, const uint64_t *tw12, bool inverse, uint32_t mode0, const uint64_t *src,
-                  uint32_t rate, bool with_coset, const uint64_t *coset);
+                  uint32_t rate, bool with_coset, const uint64_t *coset, const Tuning &tun);
+
+// Stage counts of the DIF passes over blocks of 2^L under `tun`, top
+// stages first (the last pass contiguous).
+std::vector<uint32_t> ntt_pass_sizes(uint32_t L, const Tuning &tun);
 
 // One launch over n pass descriptors in device memory (sorted by
 // tile_begin), all contiguous (s_lo == 0) or all strided.

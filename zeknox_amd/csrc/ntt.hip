@@ -34,7 +34,8 @@ void run_dif(Device &d, uint64_t *buf, uint64_t nblk, uint32_t L, uint32_t lg_tw
 {
     std::vector<NttPass> passes;
     const uint64_t *tw12 = d.twiddle_table(kNttTileLog);
-    uint64_t tiles = plan_dif(passes, buf, nblk, L, lg_tw, tw, tw12, inverse, mode0, src, rate, with_coset, coset);
+    uint64_t tiles = plan_dif(passes, buf, nblk, L, lg_tw, tw, tw12, inverse, mode0, src, rate, with_coset, coset,
+                              d.tuning_default());
     for (const NttPass &p : passes) launch_pass(d, p, tiles, p.mode == kSrcLde ? "ntt_pass_lde" : "ntt_pass");
 }
 
@@ -55,15 +56,54 @@ void launch_ntt_list(hipStream_t stream, const NttPass *dev_list, uint32_t n, ui
     ZKX_HIP(hipGetLastError());
 }
 
+// Stage counts of the passes, top stages first: every pass but the last is
+// strided (<= strided_max stages, and its stride s_lo at least its column
+// count 12 - m); the last is contiguous (<= 12 stages).
+std::vector<uint32_t> ntt_pass_sizes(uint32_t L, const Tuning &tun)
+{
+    const uint32_t smax = std::min<uint32_t>(std::max<uint32_t>(tun.ntt_strided_max, 1), kNttTileLog - 3);
+    std::vector<uint32_t> sizes;
+    if (L <= (uint32_t)kNttTileLog) {
+        sizes.push_back(L);
+        return sizes;
+    }
+    if (tun.ntt_even) {
+        const uint32_t P = std::max((L + kNttTileLog - 1) / kNttTileLog, 1 + (L - kNttTileLog + smax - 1) / smax);
+        std::vector<uint32_t> e(P, L / P);
+        for (uint32_t k = 0; k < L % P; k++) e[k]++;
+        // strided passes over their limit hand stages to the last pass
+        for (uint32_t k = 0; k + 1 < P; k++)
+            while (e[k] > smax && e[P - 1] < (uint32_t)kNttTileLog) {
+                e[k]--;
+                e[P - 1]++;
+            }
+        bool ok = e[P - 1] <= (uint32_t)kNttTileLog;
+        uint32_t below = L;
+        for (uint32_t k = 0; ok && k + 1 < P; k++) {
+            below -= e[k];
+            ok = e[k] >= 1 && e[k] <= smax && below >= kNttTileLog - e[k];
+        }
+        if (ok) return e;
+    }
+    uint32_t rest = L;
+    while (rest > (uint32_t)kNttTileLog) {
+        const uint32_t m = std::min(rest - kNttTileLog, smax);
+        sizes.push_back(m);
+        rest -= m;
+    }
+    sizes.push_back(rest);
+    return sizes;
+}
+
 uint64_t plan_dif(std::vector<NttPass> &out, uint64_t *buf, uint64_t nblk, uint32_t L, uint32_t lg_tw,
                   const uint64_t *tw, const uint64_t *tw12, bool inverse, uint32_t mode0, const uint64_t *src,
-                  uint32_t rate, bool with_coset, const uint64_t *coset)
+                  uint32_t rate, bool with_coset, const uint64_t *coset, const Tuning &tun)
 {
     const uint64_t total = nblk << L;
     const uint64_t tiles = (total + (1ull << kNttTileLog) - 1) >> kNttTileLog;
-    int hi = (int)L - 1;  // highest stage not yet planned
-    bool first = true;
-    do {
+    const std::vector<uint32_t> sizes = ntt_pass_sizes(L, tun);
+    uint32_t hi = L;  // stages [0, hi) not yet planned
+    for (size_t k = 0; k < sizes.size(); k++) {
         NttPass a{};
         a.buf = buf;
         a.src = src;
@@ -75,27 +115,15 @@ uint64_t plan_dif(std::vector<NttPass> &out, uint64_t *buf, uint64_t nblk, uint3
         a.lg_tw = lg_tw;
         a.rate = rate;
         a.inverse = inverse ? 1u : 0u;
-        a.with_coset = (with_coset && first) ? 1u : 0u;
-        a.mode = first ? mode0 : (uint32_t)kSrcInPlace;
-        uint32_t m, s_lo;
-        if (hi + 1 <= kNttTileLog) {
-            // last pass: stages hi..0 on contiguous groups (hi = -1: source mapping only)
-            m = (uint32_t)(hi + 1);
-            s_lo = 0;
-        } else {
-            // strided pass: stages hi..s_lo, at least 2^3 columns per tile, leaving
-            // <= 12 stages for the final contiguous pass
-            const int rest = hi + 1 - kNttTileLog;
-            m = (uint32_t)std::min(rest, kNttTileLog - 3);
-            s_lo = (uint32_t)(hi + 1) - m;
-        }
-        a.s_lo = s_lo;
-        a.m = m;
-        a.c = kNttTileLog - m;
+        a.with_coset = (with_coset && k == 0) ? 1u : 0u;
+        a.mode = k == 0 ? mode0 : (uint32_t)kSrcInPlace;
+        a.m = sizes[k];
+        a.s_lo = hi - a.m;  // 0 exactly for the last (contiguous) pass
+        a.c = kNttTileLog - a.m;
+        if (a.s_lo > 0 && a.s_lo < a.c) invalid("internal: NTT pass plan with stride below its column count");
         out.push_back(a);
-        first = false;
-        hi = (int)s_lo - 1;
-    } while (hi >= 0);
+        hi = a.s_lo;
+    }
     return tiles;
 }
 

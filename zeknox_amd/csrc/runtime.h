@@ -15,6 +15,8 @@
 #include <string>
 #include <vector>
 
+#include "tuning.h"
+
 namespace zkx {
 
 // Thrown inside the library, converted to RustError{code, message} at the
@@ -74,6 +76,16 @@ struct Device {
     const uint64_t *twiddle_table(uint32_t lg);  // builds on first use
     void build_coset(uint32_t lg, uint64_t gen);
     void sync();
+
+    // Schedule parameters: the default, and per commit shape (set by the
+    // caller or found by the autotuner).  Guarded by tune_mu.
+    std::mutex tune_mu;
+    Tuning tune_default;
+    std::map<TuneKey, Tuning> tuned;
+    bool autotune_new_shapes = false;  // tune unseen shapes in zkx_commit_batch
+    Tuning tuning_default();
+    // The shape's entry, or the default (found = false).
+    Tuning tuning_for(const TuneKey &k, bool *found = nullptr);
 };
 
 int device_count();

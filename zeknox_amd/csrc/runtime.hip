@@ -73,6 +73,26 @@ Device &device(size_t id)
 
 void Device::sync() { ZKX_HIP(hipStreamSynchronize(stream)); }
 
+bool valid_tuning(const Tuning &t)
+{
+    return t.ntt_strided_max >= 1 && t.ntt_strided_max <= 9 && t.ntt_even <= 1 && t.merkle_tile_log >= 8 &&
+           t.merkle_tile_log <= 10;
+}
+
+Tuning Device::tuning_default()
+{
+    std::lock_guard<std::mutex> g(tune_mu);
+    return tune_default;
+}
+
+Tuning Device::tuning_for(const TuneKey &k, bool *found)
+{
+    std::lock_guard<std::mutex> g(tune_mu);
+    auto it = tuned.find(k);
+    if (found) *found = it != tuned.end();
+    return it != tuned.end() ? it->second : tune_default;
+}
+
 const uint64_t *Device::twiddle_table(uint32_t lg)
 {
     if (lg > 32) invalid("log size " + std::to_string(lg) + " exceeds 32");
