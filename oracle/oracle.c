@@ -23,6 +23,13 @@
  *                            wrappers/go/lib/lib_test.go:197-213, reference CPU
  *   poseidon2_permute        poseidon2/poseidon2.cu:65-209
  *                            pinned: KAT tests/tests.cu:425-428, reference CPU
+ *   monolith_permute         monolith/monolith.cu:104-227 (Monolith-64)
+ *                            pinned: KAT tests/tests.cu:201-235 (hash of 0..11),
+ *                            reference CPU (oracle/_ref)
+ *   keccak_f1600 / keccak    FIPS 202 (rho offsets and round constants generated
+ *                            by their defining rules); keccak/keccak.cu:39-157
+ *                            pinned: SHA3-256 (ds 0x06) vs Python hashlib, KATs
+ *                            tests/tests.cu:162-199, reference CPU (oracle/_ref)
  *   hash_one / hash_two      poseidon/poseidon_permutation.hpp:83-139
  *   fill_digests             merkle/merkle.cpp:19-161 (linear forest layout)
  *                            pinned: Go caps KATs, reference CPU (oracle/_ref)
@@ -159,18 +166,183 @@ void orc_poseidon2_permute(u64 *s)
     }
 }
 
+/* --------------------------------------------------------------- monolith */
+
+/* One byte of Monolith-64's bar S-box: y = x ^ (rotl1(~x) & rotl2(x) &
+ * rotl3(x)), output rotl1(y) (8-bit rotations), monolith.cu:107-117. */
+static unsigned mono_sbox8(unsigned x)
+{
+#define ROTL8(v, k) ((((v) << (k)) | ((v) >> (8 - (k)))) & 0xFFu)
+    unsigned y = x ^ (ROTL8((~x) & 0xFFu, 1) & ROTL8(x, 2) & ROTL8(x, 3));
+    return ROTL8(y, 1);
+#undef ROTL8
+}
+
+/* Monolith-64 permutation, monolith.cu:119-227: concrete (rc row 0 = 0),
+ * then 6 x [bars on s[0..4) bytewise, bricks s[i] += s[i-1]^2 (integer add,
+ * carry kept as bit 64), concrete with rc row r].  concrete: row-times-matrix
+ * over the 65-bit values plus rc, reduced mod p. */
+void orc_monolith_permute(u64 *s)
+{
+    unsigned hi[12] = {0};
+    for (int r = 0; r <= 6; r++) {
+        if (r > 0) {
+            for (int i = 0; i < 4; i++) {
+                u64 v = 0;
+                for (int b = 0; b < 8; b++) v |= (u64)mono_sbox8((unsigned)(s[i] >> (8 * b)) & 0xFFu) << (8 * b);
+                s[i] = v;
+            }
+            for (int i = 11; i > 0; i--) {
+                u64 sq = orc_gl_mul(s[i - 1], s[i - 1]);
+                u64 t = s[i] + sq;
+                hi[i] = t < sq;
+                s[i] = t;
+            }
+        }
+        u64 out[12];
+        for (int row = 0; row < 12; row++) {
+            u128 acc = ZK_MONOLITH_RC[12 * r + row];
+            for (int col = 0; col < 12; col++) {
+                u128 v = ((u128)hi[col] << 64) | s[col];
+                acc += v * ZK_MONOLITH_MAT[12 * row + col];
+            }
+            out[row] = (u64)(acc % GL_P);
+        }
+        memcpy(s, out, sizeof(out));
+        memset(hi, 0, sizeof(hi));
+    }
+}
+
+/* ----------------------------------------------------------------- keccak */
+
+/* Keccak-f[1600] from its definition (FIPS 202 3.2): theta, rho, pi, chi,
+ * iota on lanes A[x + 5y].  The rho offsets and iota round constants are
+ * generated here by their defining rules (3.2.2 and the rc(t) LFSR of 3.2.5)
+ * rather than copied, so the permutation is pinned independently. */
+static void keccak_f1600(u64 *A)
+{
+    static int ready = 0;
+    static unsigned rho[25];
+    static u64 RC[24];
+    if (!ready) {
+        unsigned x = 1, y = 0;
+        rho[0] = 0;
+        for (unsigned t = 0; t < 24; t++) {
+            rho[x + 5 * y] = ((t + 1) * (t + 2) / 2) % 64;
+            unsigned nx = y, ny = (2 * x + 3 * y) % 5;
+            x = nx;
+            y = ny;
+        }
+        unsigned lfsr = 1; /* rc(t): x^8 + x^6 + x^5 + x^4 + 1 over GF(2) */
+        for (int ir = 0; ir < 24; ir++) {
+            u64 c = 0;
+            for (int j = 0; j <= 6; j++) {
+                if (lfsr & 1) c |= 1ull << ((1u << j) - 1);
+                lfsr = (lfsr & 0x80) ? ((lfsr << 1) ^ 0x171) : (lfsr << 1);
+            }
+            RC[ir] = c;
+        }
+        ready = 1;
+    }
+#define ROTL64(v, k) ((k) ? (((v) << (k)) | ((v) >> (64 - (k)))) : (v))
+    for (int ir = 0; ir < 24; ir++) {
+        u64 C[5], D[5], B[25];
+        for (int x = 0; x < 5; x++) C[x] = A[x] ^ A[x + 5] ^ A[x + 10] ^ A[x + 15] ^ A[x + 20];
+        for (int x = 0; x < 5; x++) D[x] = C[(x + 4) % 5] ^ ROTL64(C[(x + 1) % 5], 1);
+        for (int i = 0; i < 25; i++) A[i] ^= D[i % 5];
+        for (int x = 0; x < 5; x++)
+            for (int y = 0; y < 5; y++) B[y + 5 * ((2 * x + 3 * y) % 5)] = ROTL64(A[x + 5 * y], rho[x + 5 * y]);
+        for (int x = 0; x < 5; x++)
+            for (int y = 0; y < 5; y++)
+                A[x + 5 * y] = B[x + 5 * y] ^ ((~B[(x + 1) % 5 + 5 * y]) & B[(x + 2) % 5 + 5 * y]);
+        A[0] ^= RC[ir];
+    }
+#undef ROTL64
+}
+
+/* Keccak sponge over bytes with rate `rate` bytes and padding byte `ds`
+ * (0x01: the original Keccak padding the reference uses, keccak.cu:98-120;
+ * 0x06: SHA-3), output `outlen` bytes. */
+void orc_keccak(const unsigned char *in, u64 len, unsigned char *out, unsigned outlen, unsigned rate, unsigned ds)
+{
+    u64 A[25] = {0};
+    unsigned char block[200];
+    for (;;) {
+        unsigned take = len < rate ? (unsigned)len : rate;
+        memset(block, 0, sizeof(block));
+        memcpy(block, in, take);
+        if (take < rate) {
+            block[take] ^= (unsigned char)ds;
+            block[rate - 1] ^= 0x80;
+        }
+        for (unsigned i = 0; i < rate / 8; i++) {
+            u64 w = 0;
+            for (int b = 0; b < 8; b++) w |= (u64)block[8 * i + b] << (8 * b);
+            A[i] ^= w;
+        }
+        keccak_f1600(A);
+        if (take < rate) break;
+        in += rate;
+        len -= rate;
+    }
+    for (unsigned i = 0; i < outlen; i++) out[i] = (unsigned char)(A[i / 8] >> (8 * (i % 8)));
+}
+
+/* KeccakHasher, keccak.cu:136-157: Keccak-256 of the input words' bytes,
+ * keeping 25 bytes (the top 7 bytes of word 3 cleared); fewer than 4 words
+ * are copied (zero-padded here; the reference leaves the rest of its output
+ * buffer as it was).  hash_two hashes the first 25 bytes of each digest. */
+static void keccak_hash_one(const u64 *in, u64 n, u64 *out)
+{
+    if (n < 4) {
+        for (u64 i = 0; i < 4; i++) out[i] = i < n ? in[i] : 0;
+        return;
+    }
+    unsigned char *bytes = (unsigned char *)malloc(n * 8), md[32];
+    for (u64 i = 0; i < n * 8; i++) bytes[i] = (unsigned char)(in[i / 8] >> (8 * (i % 8)));
+    orc_keccak(bytes, n * 8, md, 32, 136, 0x01);
+    free(bytes);
+    for (int i = 0; i < 4; i++) {
+        u64 w = 0;
+        for (int b = 0; b < 8; b++) w |= (u64)md[8 * i + b] << (8 * b);
+        out[i] = w;
+    }
+    out[3] &= 0xFF;
+}
+
+static void keccak_hash_two(const u64 *l, const u64 *r, u64 *out)
+{
+    unsigned char bytes[50], md[32];
+    for (int i = 0; i < 25; i++) {
+        bytes[i] = (unsigned char)(l[i / 8] >> (8 * (i % 8)));
+        bytes[25 + i] = (unsigned char)(r[i / 8] >> (8 * (i % 8)));
+    }
+    orc_keccak(bytes, 50, md, 32, 136, 0x01);
+    for (int i = 0; i < 4; i++) {
+        u64 w = 0;
+        for (int b = 0; b < 8; b++) w |= (u64)md[8 * i + b] << (8 * b);
+        out[i] = w;
+    }
+    out[3] &= 0xFF;
+}
+
 /* ---------------------------------------------------------------- sponge */
 
-enum { HASH_POSEIDON = 0, HASH_POSEIDON2 = 3 };
+enum { HASH_POSEIDON = 0, HASH_KECCAK = 1, HASH_POSEIDON2 = 3, HASH_MONOLITH = 4 };
 
 static void permute(u64 *s, int hash_type)
 {
     if (hash_type == HASH_POSEIDON2) orc_poseidon2_permute(s);
+    else if (hash_type == HASH_MONOLITH) orc_monolith_permute(s);
     else orc_poseidon_permute(s);
 }
 
 void orc_hash_one(const u64 *in, u64 n, u64 *out, int hash_type)
 {
+    if (hash_type == HASH_KECCAK) {
+        keccak_hash_one(in, n, out);
+        return;
+    }
     if (n <= 4) {
         for (u64 i = 0; i < 4; i++) out[i] = i < n ? in[i] : 0;
         return;
@@ -186,6 +358,10 @@ void orc_hash_one(const u64 *in, u64 n, u64 *out, int hash_type)
 
 void orc_hash_two(const u64 *l, const u64 *r, u64 *out, int hash_type)
 {
+    if (hash_type == HASH_KECCAK) {
+        keccak_hash_two(l, r, out);
+        return;
+    }
     u64 s[12] = {0};
     memcpy(s, l, 4 * sizeof(u64));
     memcpy(s + 4, r, 4 * sizeof(u64));

@@ -33,9 +33,12 @@ MERKLE = [  # (log2 leaves, leaf size, cap height, hash, seed)
     (3, 7, 3, 0, 11), (3, 7, 2, 0, 12), (4, 7, 1, 0, 13), (5, 9, 2, 3, 14), (6, 2, 4, 0, 15),
     (6, 4, 0, 3, 16), (7, 5, 3, 0, 17), (8, 8, 4, 3, 18), (8, 20, 4, 0, 19), (9, 135, 4, 3, 20),
     (10, 80, 4, 0, 21), (10, 1, 2, 3, 22), (11, 13, 9, 0, 23), (12, 135, 4, 0, 24),
+    # Keccak-256 (1) and Monolith (4) trees
+    (3, 7, 1, 1, 31), (5, 17, 2, 1, 32), (6, 4, 0, 1, 33), (8, 135, 4, 1, 34), (4, 2, 4, 1, 35),
+    (3, 9, 1, 4, 41), (6, 12, 3, 4, 42), (8, 135, 4, 4, 43), (5, 4, 5, 4, 44),
 ]
-LARGE = [  # hashed outputs: 2^13 x 135, cap height 4, both hashes
-    (13, 135, 4, 0, 1001), (13, 135, 4, 3, 1002),
+LARGE = [  # hashed outputs: 2^13 x 135, cap height 4, every hash
+    (13, 135, 4, 0, 1001), (13, 135, 4, 3, 1002), (13, 135, 4, 1, 1003), (13, 135, 4, 4, 1004),
 ]
 LDE = [  # (lg_n, rate, cols, coset, seed)
     (4, 1, 3, True, 31), (6, 2, 2, True, 32), (8, 3, 4, False, 33), (10, 1, 5, True, 34),

@@ -41,7 +41,8 @@ def oracle_commit(coeffs, cols, lg_n, rate, h, ht):
 
 # (cols, lg_n, rate, cap_height, hash)
 MIX = [(2, 10, 1, 4, 0), (20, 11, 2, 4, 3), (80, 10, 3, 4, 0), (135, 9, 1, 4, 3), (5, 3, 1, 0, 0),
-       (7, 2, 2, 4, 0), (3, 1, 1, 2, 3), (4, 5, 2, 7, 0), (9, 12, 1, 2, 3), (1, 6, 1, 1, 0), (135, 13, 1, 4, 0)]
+       (7, 2, 2, 4, 0), (3, 1, 1, 2, 3), (4, 5, 2, 7, 0), (9, 12, 1, 2, 3), (1, 6, 1, 1, 0), (135, 13, 1, 4, 0),
+       (17, 9, 2, 4, 1), (34, 8, 1, 3, 4), (3, 7, 1, 2, 1), (12, 6, 3, 4, 4)]
 
 
 def test_batch_mixed_vs_oracle(zk):

@@ -25,6 +25,29 @@ def test_poseidon2_kat_direct(zk):
     assert [int(v) for v in caps] == POSEIDON2_KAT
 
 
+def test_keccak_kat_direct(zk):
+    """tests/tests.cu:162-199: Keccak hash_one of the first `size` words."""
+    data = [13421290117754017454, 7401888676587830362, 15316685236050041751, 13588825262671526271,
+            13421290117754017454, 7401888676587830362]
+    expected = {
+        1: [13421290117754017454, 0, 0, 0],
+        2: [13421290117754017454, 7401888676587830362, 0, 0],
+        3: [13421290117754017454, 7401888676587830362, 15316685236050041751, 0],
+        4: [9981707860959651334, 16351366398560378420, 4283762868800363615, 101],
+        5: [708367124667950404, 17208681281141108820, 8334320481120086961, 134],
+        6: [16109761546392287110, 4918745475135463511, 17110319063854316944, 103],
+    }
+    for size, want in expected.items():
+        _, caps = gpu_tree(zk, np.array(data[:size], np.uint64), size, 0, 1)
+        assert [int(v) for v in caps] == want
+
+
+def test_monolith_kat_direct(zk):
+    """tests/tests.cu:201-235: Monolith hash_one(0..11)."""
+    _, caps = gpu_tree(zk, np.arange(12, dtype=np.uint64), 12, 0, 4)
+    assert [int(v) for v in caps] == [0xCB4EF9B3FE5BCA9E, 0xE03C9506D19C8216, 0x2F05CFB355E880C, 0xF614E84BF4DF8342]
+
+
 @pytest.mark.parametrize("nl,nc,cap", GO_CAPS)
 def test_go_cap_kats(zk, nl, nc, cap):
     leaves = np.array(GO_LEAF7 * nl, dtype=np.uint64)
@@ -37,7 +60,7 @@ SHAPES = [(1, 7, 0), (1, 7, 1), (2, 9, 1), (3, 5, 1), (4, 2, 2), (5, 4, 3), (6, 
           (8, 17, 8), (9, 1, 4), (10, 135, 4), (11, 20, 0), (12, 80, 4), (9, 12, 9), (10, 400, 2)]
 
 
-@pytest.mark.parametrize("ht", [0, 3])
+@pytest.mark.parametrize("ht", [0, 1, 3, 4])
 @pytest.mark.parametrize("lg_l,ls,h", SHAPES)
 def test_tree_vs_oracle(zk, ht, lg_l, ls, h):
     print(f"tree lg_l={lg_l} ls={ls} h={h} ht={ht}", flush=True)
@@ -48,7 +71,7 @@ def test_tree_vs_oracle(zk, ht, lg_l, ls, h):
     assert np.array_equal(dig, edig)
 
 
-@pytest.mark.parametrize("ht", [0, 3])
+@pytest.mark.parametrize("ht", [0, 1, 3, 4])
 @pytest.mark.parametrize("lg_l,ls,h", [(13, 135, 4),   # 2^13 x 135 leaves, cap height 4
                                        (17, 5, 3),     # two passes over HBM
                                        (18, 5, 0)])    # three passes over HBM
@@ -83,5 +106,5 @@ def test_invalid_arguments_are_reported(zk):
     with pytest.raises(zk.ZeknoxError):  # leaves not a power of two
         zk.fill_digests_buf_linear_gpu_with_gpu_ptr(d, c, l, 2 * (24 - 4), 4, 24, 8, 2, 0)
     with pytest.raises(zk.ZeknoxError):  # hash type not provided
-        zk.fill_digests_buf_linear_gpu_with_gpu_ptr(d, c, l, 2 * (32 - 4), 4, 32, 8, 2, 1)
+        zk.fill_digests_buf_linear_gpu_with_gpu_ptr(d, c, l, 2 * (32 - 4), 4, 32, 8, 2, 2)
     assert np.all(c.to_host() == 0)  # nothing written

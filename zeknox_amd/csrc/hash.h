@@ -8,15 +8,18 @@
 // Poseidon      : poseidon/poseidon.cu:563-838 (naive rounds; same function
 //                 as the fast-partial-round form of poseidon/poseidon.cpp).
 // Poseidon2     : poseidon2/poseidon2.cu:65-209.
+// Monolith      : monolith/monolith.cu:104-227 (monolith.h).
+// Keccak-256    : keccak/keccak.cu (keccak.h; its own sponge).
 // Sponge        : poseidon/poseidon_permutation.hpp:83-139 /
 //                 poseidon_permutation.cuh:83-134.
 #pragma once
 #include "gl64.h"
 #include "gl_constants.h"
+#include "keccak.h"
+#include "merkle_types.h"
+#include "monolith.h"
 
 namespace zkx {
-
-enum HashKind : int { kPoseidon = 0, kPoseidon2 = 3 };
 
 // Round constants, [round][12].  The Poseidon table has one extra all-zero
 // row: after round r the next round's constants (row r+1) are folded into
@@ -187,6 +190,7 @@ template <int HT>
 __device__ __forceinline__ void permute(uint64_t s[12])
 {
     if constexpr (HT == kPoseidon2) poseidon2_permute(s);
+    else if constexpr (HT == kMonolith) monolith_permute(s);
     else poseidon_permute(s);
 }
 
@@ -194,6 +198,10 @@ __device__ __forceinline__ void permute(uint64_t s[12])
 template <int HT>
 __device__ __forceinline__ void hash_two(const uint64_t l[4], const uint64_t r[4], uint64_t out[4])
 {
+    if constexpr (HT == kKeccak) {
+        keccak_hash_two(l, r, out);
+        return;
+    }
     uint64_t s[12];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -212,6 +220,10 @@ __device__ __forceinline__ void hash_two(const uint64_t l[4], const uint64_t r[4
 template <int HT, class Load>
 __device__ __forceinline__ void hash_one(uint32_t n, Load load, uint64_t out[4])
 {
+    if constexpr (HT == kKeccak) {
+        keccak_hash_one(n, load, out);
+        return;
+    }
     if (n <= 4) {
 #pragma unroll
         for (int i = 0; i < 4; i++) out[i] = (uint32_t)i < n ? load(i) : 0;

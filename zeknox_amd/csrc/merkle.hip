@@ -6,7 +6,7 @@
 #include <algorithm>
 #include <string>
 
-#include "merkle_kernels.h"
+#include "merkle_launch.h"
 
 namespace zkx {
 
@@ -19,8 +19,9 @@ uint32_t log2u(uint64_t x) { return 63 - __builtin_clzll(x); }
 
 void check_merkle(const MerkleSpec &s)
 {
-    if (s.hash_type != 0 && s.hash_type != 3)
-        invalid("hash type " + std::to_string(s.hash_type) + " is not provided (Poseidon = 0, Poseidon2 = 3)");
+    if (s.hash_type != kPoseidon && s.hash_type != kKeccak && s.hash_type != kPoseidon2 && s.hash_type != kMonolith)
+        invalid("hash type " + std::to_string(s.hash_type) +
+                " is not provided (Poseidon = 0, Keccak = 1, Poseidon2 = 3, Monolith = 4)");
     if (!s.caps || !s.leaves) invalid("null cap or leaf buffer");
     if (!is_pow2(s.num_leaves) || !is_pow2(s.num_caps) || s.num_caps > s.num_leaves)
         invalid("leaves and caps must be powers of two with caps <= leaves");
@@ -64,28 +65,6 @@ void plan_merkle(const MerkleSpec &s, std::vector<MerkleArgs> &passes, uint32_t 
     }
 }
 
-namespace {
-
-template <int HT, int TPB>
-void launch_tpb(hipStream_t stream, bool from_leaves, const MerkleArgs *dev_list, uint32_t n, const MerkleArgs &one,
-                uint64_t blocks)
-{
-    const dim3 g((uint32_t)blocks), b(TPB);
-    if (from_leaves) hipLaunchKernelGGL((k_merkle_tile<HT, true, TPB>), g, b, 0, stream, dev_list, n, one);
-    else hipLaunchKernelGGL((k_merkle_tile<HT, false, TPB>), g, b, 0, stream, dev_list, n, one);
-}
-
-template <int HT>
-void launch_hash(hipStream_t stream, bool from_leaves, const MerkleArgs *dev_list, uint32_t n, const MerkleArgs &one,
-                 uint64_t blocks, uint32_t tile_log_max)
-{
-    if (tile_log_max == 10) launch_tpb<HT, 1024>(stream, from_leaves, dev_list, n, one, blocks);
-    else if (tile_log_max == 9) launch_tpb<HT, 512>(stream, from_leaves, dev_list, n, one, blocks);
-    else launch_tpb<HT, 256>(stream, from_leaves, dev_list, n, one, blocks);
-}
-
-}  // namespace
-
 void launch_merkle_pass(hipStream_t stream, int hash_type, bool from_leaves, const MerkleArgs *dev_list,
                         uint32_t n, const MerkleArgs &one, uint64_t blocks, uint32_t tile_log_max)
 {
@@ -94,8 +73,12 @@ void launch_merkle_pass(hipStream_t stream, int hash_type, bool from_leaves, con
     if (tile_log_max < (uint32_t)kMerkleMinTileLog || tile_log_max > (uint32_t)kMerkleMaxTileLog)
         invalid("Merkle tile size out of range");
     ProfScope ps(from_leaves ? "merkle_leaves" : "merkle_levels", stream);
-    if (hash_type == kPoseidon2) launch_hash<kPoseidon2>(stream, from_leaves, dev_list, n, one, blocks, tile_log_max);
-    else launch_hash<kPoseidon>(stream, from_leaves, dev_list, n, one, blocks, tile_log_max);
+    switch (hash_type) {
+    case kPoseidon2: launch_merkle_hash<kPoseidon2>(stream, from_leaves, dev_list, n, one, blocks, tile_log_max); break;
+    case kKeccak: launch_merkle_hash<kKeccak>(stream, from_leaves, dev_list, n, one, blocks, tile_log_max); break;
+    case kMonolith: launch_merkle_hash<kMonolith>(stream, from_leaves, dev_list, n, one, blocks, tile_log_max); break;
+    default: launch_merkle_hash<kPoseidon>(stream, from_leaves, dev_list, n, one, blocks, tile_log_max); break;
+    }
     ZKX_HIP(hipGetLastError());
 }
 
