@@ -27,7 +27,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# config-2 shape
+# the headline commit: 135 polys x 2^15, rate_bits 3, cap height 4
 LG_N, RATE, COLS, CAP_H = 15, 3, 135, 4
 LG = LG_N + RATE
 N = 1 << LG
@@ -36,9 +36,22 @@ HASH = 0  # Poseidon
 # MI355X published peaks
 HBM_PEAK_GBS = 8000.0
 
-# VALU wave-instructions of one config-2 commit's leaf-hash launch
-# (rocprofv3 --pmc SQ_INSTS_VALU; profiles/merkle_leaves_valu_config2.txt)
-VALU_WAVE_INSTS_PER_COMMIT = {0: 2.13e9, 3: 1.6e9}
+# Per-unit figures of the leaf-hash launch on this workload, measured with
+# rocprofv3 --pmc on the batched 135 x 2^15 rate-3 launch (4 commits, 256-lane
+# work-groups) and kept in profiles/merkle_leaves_valu_135x2e18.txt and
+# profiles/hbm_traffic_135x2e15_rate3.txt:
+#   VALU wave-instructions per Poseidon permutation (SQ_INSTS_VALU / perms,
+#   leaf loads and in-tile levels included)
+VALU_WAVE_INSTS_PER_PERM = 372.3
+#   HBM bytes per commit: FETCH_SIZE x 2 (the counter reads half the bytes of
+#   a plain 1 GiB copy) + WRITE_SIZE x 1, KiB x 1024
+LEAF_TRAFFIC_BYTES_PER_COMMIT = 7.75e8
+#   NTT passes (both), same scaling
+NTT_TRAFFIC_BYTES_PER_COMMIT = 8.87e8
+# 32-bit VALU issue peak: 256 CUs x 4 SIMDs x one wave64 instruction per 4
+# cycles x 2.4 GHz = 39.3 T lane-ops/s (MI355X's published 78.6 TFLOP/s
+# vector FP64 counts an FMA as 2 flops on the same issue rate)
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 
 
 def parse():
@@ -46,13 +59,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--jobs", type=int, default=4, help="config-2 commits per batched step")
+    ap.add_argument("--jobs", type=int, default=4, help="135 x 2^15 rate-3 commits per batched step")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="served-stream duration (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=40.0, help="served-stream arrivals per second")
     ap.add_argument("--no-tune", action="store_true", help="batched path on the default schedule (no autotuning)")
     ap.add_argument("--schedule", default=None, metavar="S,E,T",
-                    help="batched config-2 schedule (ntt_strided_max,ntt_even,merkle_tile_log) instead of autotuning")
+                    help="batched 135 x 2^15 schedule (ntt_strided_max,ntt_even,merkle_tile_log) instead of autotuning")
     ap.add_argument("--no-extra", action="store_true", help="skip the 64-job and 8 x 400 x 2^20 batches")
     ap.add_argument("--extra-steps", type=int, default=3, help="timed steps of those batches")
     ap.add_argument("--cpu-runs", type=int, default=5, help="whole CPU commits timed (median reported)")
@@ -62,7 +75,7 @@ def parse():
 
 
 def cpu_baseline(np, coeffs, runs, gpu_caps):
-    """The same config-2 commit on the host cores, `runs` times, median
+    """The same 135 x 2^15 rate-3 commit on the host cores, `runs` times, median
     reported.  Merkle: the reference's own CPU fill_digests_buf_linear_cpu
     (native/merkle/merkle.cpp, OpenMP, compiled from its sources into
     oracle/_ref) when that build is present, else the oracle's restatement of
@@ -100,7 +113,7 @@ def cpu_baseline(np, coeffs, runs, gpu_caps):
         pass
     return {"value": round(1.0 / statistics.median(tot), 4), "unit": "commits/s",
             "cores": ob.orc.orc_num_threads(), "kind": "reference" if ref is not None else "port",
-            "sample": f"{runs} whole config-2 commits of the GPU run's input, median: LDE+transpose "
+            "sample": f"{runs} whole 135 x 2^15 rate-3 commits of the GPU run's input, median: LDE+transpose "
                       f"{statistics.median(lde_t):.2f} s (oracle C, OpenMP), Poseidon Merkle "
                       f"{statistics.median(mer_t):.2f} s ("
                       + ("reference merkle.cpp compiled from its sources" if ref is not None else "oracle C")
@@ -172,7 +185,7 @@ def plain_vs_batched(zk, np, shapes, steps, warmup, seed, tune):
             "schedules (ntt_strided_max, ntt_even, merkle_tile_log)": [list(t) for t in schedules]}
 
 
-def config3(zk, np, steps, warmup, tune, seed=3):
+def batch64_mixed(zk, np, steps, warmup, tune, seed=3):
     rng = np.random.default_rng(seed)
     shapes = [(int(rng.choice([2, 20, 80, 135])), int(rng.integers(10, 17)), int(rng.integers(1, 4)), 0)
               for _ in range(64)]
@@ -182,7 +195,7 @@ def config3(zk, np, steps, warmup, tune, seed=3):
     return r
 
 
-def config4(zk, np, steps, warmup, tune):
+def large_poseidon2_batch(zk, np, steps, warmup, tune):
     r = plain_vs_batched(zk, np, [(400, 20, 1, 3)] * 8, steps, warmup, 4, tune)
     r["workload"] = "8 jobs of 400 polys x 2^20, rate_bits 1 (2^21-point LDE), Poseidon2, cap_height 4, all resident"
     return r
@@ -331,7 +344,7 @@ def main():
     kernels = {k: {"launches_per_commit": n / args.steps, "ms_per_commit": ms / args.steps,
                    "avg_launch_ms": ms / n} for k, (n, ms) in stats.items()}
 
-    # ---- batched path: JOBS config-2 commits per zkx_commit_batch call
+    # ---- batched path: JOBS such commits per zkx_commit_batch call
     commits = [zk.Commit(d_in, COLS, LG_N, RATE, CAP_H, HASH) for _ in range(args.jobs)]
     if args.schedule:
         zk.set_tuning(zk.Tuning(*[int(v) for v in args.schedule.split(",")]), commits[0])
@@ -364,22 +377,22 @@ def main():
 
     # ---- roofline of the dominant kernel: the leaf-hash launch of the batch
     # (Poseidon permutations, VALU-bound: 64-bit modular arithmetic on the
-    # 32-bit integer VALU, no MFMA).  achieved = VALU issue slots per launch
-    # (SQ_INSTS_VALU of that launch x 64 lanes, measured with rocprofv3 --pmc
-    # on this workload: profiles/merkle_leaves_valu_config2.txt) / the
-    # launch's HIP-event duration; peak = 256 CUs x 64 lanes x 2.4 GHz.
+    # 32-bit integer VALU, no MFMA).  achieved = permutations per launch x
+    # VALU lane-instructions per permutation / the launch's HIP-event duration
+    # (events on the library's stream); peak = VALU_PEAK_TOPS.
     dom = max(batch_kernels, key=lambda k: batch_kernels[k]["ms_per_step"])
     leaf = batch_kernels.get("merkle_leaves")
-    valu_peak = 256 * 64 * 2.4e9 / 1e12  # TOP/s
     roofline = None
     if leaf:
-        insts = VALU_WAVE_INSTS_PER_COMMIT[HASH] * args.jobs
-        achieved = insts * 64 / (leaf["avg_launch_ms"] * 1e-3) / 1e12
-        roofline = {"bound": "valu", "achieved": round(achieved, 2), "peak": round(valu_peak, 2),
-                    "unit": "TOP/s (32-bit VALU lane-ops)", "frac": round(achieved / valu_peak, 3),
-                    "traffic": None, "kernel": "merkle_leaves",
-                    "avg_launch_ms": round(leaf["avg_launch_ms"], 4),
-                    "perms_per_launch": args.jobs * perms_per_commit(N, COLS, nc)}
+        perms = args.jobs * perms_per_commit(N, COLS, nc)
+        achieved = perms * VALU_WAVE_INSTS_PER_PERM * 64 / (leaf["avg_launch_ms"] * 1e-3) / 1e12
+        alg_bytes = args.jobs * (8 * COLS * N * 2 + 32 * nd)  # LDE read, leaf rows + digests written
+        roofline = {"bound": "valu", "achieved": round(achieved, 2), "peak": round(VALU_PEAK_TOPS, 2),
+                    "unit": "TOP/s (32-bit VALU lane-ops)", "frac": round(achieved / VALU_PEAK_TOPS, 3),
+                    "traffic": round(LEAF_TRAFFIC_BYTES_PER_COMMIT * args.jobs), "algorithmic_bytes": alg_bytes,
+                    "hbm_gbs": round(LEAF_TRAFFIC_BYTES_PER_COMMIT * args.jobs / (leaf["avg_launch_ms"] * 1e-3) / 1e9, 1),
+                    "kernel": "merkle_leaves", "avg_launch_ms": round(leaf["avg_launch_ms"], 4),
+                    "perms_per_launch": perms, "valu_wave_insts_per_perm": VALU_WAVE_INSTS_PER_PERM}
 
     # LDE passes against HBM: algorithmic bytes of the LDE (read n, write N
     # per column) over the batch's NTT launches
@@ -389,13 +402,16 @@ def main():
         lde_bytes = 8 * COLS * ((1 << LG_N) + N) * args.jobs
         gbs = lde_bytes / (ntt["ms_per_step"] * 1e-3) / 1e9
         ntt_roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(gbs / HBM_PEAK_GBS, 3), "traffic": None, "kernel": "batch_ntt (all passes)"}
+                        "frac": round(gbs / HBM_PEAK_GBS, 3), "traffic": round(NTT_TRAFFIC_BYTES_PER_COMMIT * args.jobs),
+                        "traffic_gbs": round(NTT_TRAFFIC_BYTES_PER_COMMIT * args.jobs / (ntt["ms_per_step"] * 1e-3) / 1e9, 1),
+                        "kernel": "batch_ntt (all passes)",
+                        "note": "VALU-bound (64-bit modular butterflies), not HBM-bound"}
 
     # ---- heterogeneous 64-job batch and the large Poseidon2 batch, both ways
     extra = {}
     if not args.no_extra:
-        extra["batch64"] = config3(zk, np, args.extra_steps, 1, not args.no_tune)
-        extra["large_poseidon2"] = config4(zk, np, args.extra_steps, 1, not args.no_tune)
+        extra["batch64"] = batch64_mixed(zk, np, args.extra_steps, 1, not args.no_tune)
+        extra["large_poseidon2"] = large_poseidon2_batch(zk, np, args.extra_steps, 1, not args.no_tune)
 
     # ---- CPU baseline: whole commits of the same input on the host cores
     cpu = None

@@ -99,7 +99,7 @@ def test_batch_without_leaf_rows(zk):
 def test_batch_rejects_bad_jobs(zk):
     x = ob.canonical(8 << 4, 1)
     bad = zk.Commit(x, 8, 4, 1, 3, 0)
-    bad.hash_type = 1
+    bad.hash_type = 2  # Poseidon-BN128: not provided
     with pytest.raises(zk.ZeknoxError):
         zk.commit_batch([bad])
     bad.hash_type = 0

@@ -79,7 +79,7 @@ def test_transpose_rev_vs_oracle(zk, lg, B):
     assert np.array_equal(out, ob.transpose_rev(x, lg, B))
 
 
-def test_lde_config2_shape_vs_oracle(zk):
+def test_lde_135x2e15_rate3_vs_oracle(zk):
     """135 columns of 2^15 coefficients, rate 3, coset 7 -> 2^18-point LDE."""
     lg_n, rate, B = 15, 3, 135
     x = ob.canonical(B << lg_n, 2024)

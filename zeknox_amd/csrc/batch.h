@@ -78,10 +78,12 @@ Engine &default_engine(Device &d);
 
 // Shape key of a job for the device's tuning table.
 TuneKey tune_key(const CommitJob &j);
-// Times the job on `e` under each distinct schedule (NTT pass split, then
-// Merkle work-group size; `iters` runs each, fastest kept), records the best
-// for the job's shape on the device and returns it.  The job's outputs are
-// written (every schedule computes the same result).  Call with e.mu held.
-Tuning autotune(Engine &e, const CommitJob &job, int iters, void *scratch = nullptr);
+// Times the job's shape on `e` under each distinct schedule (NTT pass split,
+// then Merkle work-group size; `iters` runs each of a GPU-filling batch of
+// copies, fastest kept), records the best for the shape on the device and
+// returns it.  The job's outputs are written (every schedule computes the
+// same result); the engine's own working buffer grows to the batch's size.
+// Call with e.mu held.
+Tuning autotune(Engine &e, const CommitJob &job, int iters);
 
 }  // namespace zkx

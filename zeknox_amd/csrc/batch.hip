@@ -84,7 +84,7 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
         }
         bool found = false;
         tun[i] = dev_.tuning_for(tune_key(jobs[i]), &found);
-        if (!found && dev_.autotune_new_shapes) tun[i] = autotune(*this, jobs[i], 3, scratch);
+        if (!found && dev_.autotune_new_shapes) tun[i] = autotune(*this, jobs[i], 3);
     }
 
     // ---- working memory: every job's LDE in DIF (bit-reversed) order

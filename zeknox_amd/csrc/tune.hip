@@ -5,7 +5,13 @@
 // of at most 4..9 stages, greedy or evenly spread) and how many leaves one
 // Merkle work-group hashes and reduces in LDS (2^8..2^10).  Candidates that
 // give the same launch plan are timed once.  Times are HIP events on the
-// engine's stream around whole commits of the job, fastest of `iters`.
+// engine's stream around whole commits, fastest of `iters`.
+//
+// A schedule is timed on a batch of copies of the job large enough to fill
+// the GPU (at least 2^20 leaf rows, at most 64 copies): that is how shapes
+// run in a batch, and a lone small job would favour work-group sizes that
+// only pay off when the GPU is half empty.  The copies share the job's
+// outputs (they all write the same values).
 #include <algorithm>
 #include <vector>
 
@@ -34,12 +40,12 @@ struct Timer {
     }
 };
 
-double time_commit(Engine &e, const CommitJob &job, const Tuning &t, int iters, void *scratch, Timer &tm)
+double time_commit(Engine &e, const std::vector<CommitJob> &jobs, const Tuning &t, int iters, Timer &tm)
 {
     double best = 1e30;
     for (int k = 0; k < iters; k++) {
         ZKX_HIP(hipEventRecord(tm.a, e.stream()));
-        e.commit(&job, 1, scratch, &t);
+        e.commit(jobs.data(), jobs.size(), nullptr, &t);
         ZKX_HIP(hipEventRecord(tm.b, e.stream()));
         ZKX_HIP(hipEventSynchronize(tm.b));
         float ms = 0.f;
@@ -51,16 +57,19 @@ double time_commit(Engine &e, const CommitJob &job, const Tuning &t, int iters, 
 
 }  // namespace
 
-Tuning autotune(Engine &e, const CommitJob &job, int iters, void *scratch)
+Tuning autotune(Engine &e, const CommitJob &job, int iters)
 {
     check_commit_job(job);
     iters = std::max(iters, 1);
     Device &d = e.device();
     Timer tm;
+    const uint32_t lg_rows = job.lg_n + job.rate;
+    const size_t copies = lg_rows >= 20 ? 1 : std::min<size_t>(64, (size_t)1 << (20 - lg_rows));
+    const std::vector<CommitJob> batch(copies, job);
     Tuning best = d.tuning_default();
-    // warm-up: builds the tables the shape needs
-    e.commit(&job, 1, scratch, &best);
-    double best_ms = time_commit(e, job, best, iters, scratch, tm);
+    // warm-up: builds the tables the shape needs and sizes the engine's scratch
+    e.commit(batch.data(), batch.size(), nullptr, &best);
+    double best_ms = time_commit(e, batch, best, iters, tm);
 
     // NTT pass split over the coefficient blocks (2^lg_n points)
     std::vector<std::vector<uint32_t>> seen{ntt_pass_sizes(job.lg_n, best)};
@@ -72,7 +81,7 @@ Tuning autotune(Engine &e, const CommitJob &job, int iters, void *scratch)
             const std::vector<uint32_t> plan = ntt_pass_sizes(job.lg_n, c);
             if (std::find(seen.begin(), seen.end(), plan) != seen.end()) continue;
             seen.push_back(plan);
-            const double ms = time_commit(e, job, c, iters, scratch, tm);
+            const double ms = time_commit(e, batch, c, iters, tm);
             if (ms < best_ms) {
                 best_ms = ms;
                 best = c;
@@ -86,7 +95,7 @@ Tuning autotune(Engine &e, const CommitJob &job, int iters, void *scratch)
         if (tl == ntt_best.merkle_tile_log || (tl > lg_leaves && ntt_best.merkle_tile_log >= lg_leaves)) continue;
         Tuning c = ntt_best;
         c.merkle_tile_log = tl;
-        const double ms = time_commit(e, job, c, iters, scratch, tm);
+        const double ms = time_commit(e, batch, c, iters, tm);
         if (ms < best_ms) {
             best_ms = ms;
             best = c;
