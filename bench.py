@@ -289,7 +289,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build drives one GPU; --gpus {args.gpus} is not supported "
-              "(jobs would shard across GPUs by commit, see DESIGN.md)", file=sys.stderr)
+              "(commit jobs are independent and would shard across GPUs one process per GPU)", file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
