@@ -30,6 +30,10 @@
  *                            by their defining rules); keccak/keccak.cu:39-157
  *                            pinned: SHA3-256 (ds 0x06) vs Python hashlib, KATs
  *                            tests/tests.cu:162-199, reference CPU (oracle/_ref)
+ *   bn_mont_mul / bn_add     poseidon/element_bn128.cu (BN254 Fr, Montgomery R = 2^256)
+ *                            pinned: Python big integers (a*b*R^-1 mod r, a+b mod r)
+ *   poseidon_bn128_permute   poseidon/poseidon_bn128.cu:116-204 (t = 5, x^5, optimised rounds)
+ *                            pinned: KAT tests/tests.cu:492-536, reference CPU (oracle/_ref)
  *   hash_one / hash_two      poseidon/poseidon_permutation.hpp:83-139
  *   fill_digests             merkle/merkle.cpp:19-161 (linear forest layout)
  *                            pinned: Go caps KATs, reference CPU (oracle/_ref)
@@ -326,14 +330,180 @@ static void keccak_hash_two(const u64 *l, const u64 *r, u64 *out)
     out[3] &= 0xFF;
 }
 
+/* ---------------------------------------------------------- bn254 poseidon */
+
+/* The BN254 scalar field r (four 64-bit limbs, least significant first) and
+ * Montgomery multiplication with R = 2^256 by the CIOS method (Koc, Acar,
+ * Kaliski 1996).  R^2 mod r and -r^-1 mod 2^64 are computed here from r. */
+static const u64 BN_R[4] = {0x43e1f593f0000001ull, 0x2833e84879b97091ull, 0xb85045b68181585dull,
+                            0x30644e72e131a029ull};
+
+static int bn_geq(const u64 *a, const u64 *b)
+{
+    for (int i = 3; i >= 0; i--)
+        if (a[i] != b[i]) return a[i] > b[i];
+    return 1;
+}
+
+static void bn_sub_r(u64 *a)
+{
+    u128 borrow = 0;
+    for (int i = 0; i < 4; i++) {
+        u128 d = (u128)a[i] - BN_R[i] - borrow;
+        a[i] = (u64)d;
+        borrow = (d >> 64) ? 1 : 0;
+    }
+}
+
+/* z = a + b mod r for a, b < r */
+void orc_bn_add(u64 *z, const u64 *a, const u64 *b)
+{
+    u128 c = 0;
+    u64 t[4];
+    for (int i = 0; i < 4; i++) {
+        c += (u128)a[i] + b[i];
+        t[i] = (u64)c;
+        c >>= 64;
+    }
+    if (c || bn_geq(t, BN_R)) bn_sub_r(t);
+    memcpy(z, t, sizeof(t));
+}
+
+static u64 bn_ninv(void)
+{
+    /* -r^-1 mod 2^64 by Newton iteration on the inverse of r[0] (odd) */
+    u64 x = 1;
+    for (int i = 0; i < 7; i++) x *= 2 - BN_R[0] * x;
+    return (u64)0 - x;
+}
+
+/* z = a * b * R^-1 mod r for a, b < r */
+void orc_bn_mont_mul(u64 *z, const u64 *a, const u64 *b)
+{
+    const u64 ninv = bn_ninv();
+    u64 t[6] = {0};
+    for (int i = 0; i < 4; i++) {
+        u128 c = 0;
+        for (int j = 0; j < 4; j++) {
+            c += (u128)t[j] + (u128)a[i] * b[j];
+            t[j] = (u64)c;
+            c >>= 64;
+        }
+        c += t[4];
+        t[4] = (u64)c;
+        t[5] = (u64)(c >> 64);
+        const u64 m = t[0] * ninv;
+        c = (u128)t[0] + (u128)m * BN_R[0];
+        c >>= 64;
+        for (int j = 1; j < 4; j++) {
+            c += (u128)t[j] + (u128)m * BN_R[j];
+            t[j - 1] = (u64)c;
+            c >>= 64;
+        }
+        c += t[4];
+        t[3] = (u64)c;
+        t[4] = t[5] + (u64)(c >> 64);
+    }
+    if (t[4] || bn_geq(t, BN_R)) bn_sub_r(t);
+    memcpy(z, t, 4 * sizeof(u64));
+}
+
+static void bn_r2(u64 *out) /* R^2 mod r = 2^512 mod r, by doubling */
+{
+    u64 x[4] = {1, 0, 0, 0};
+    for (int i = 0; i < 512; i++) orc_bn_add(x, x, x);
+    memcpy(out, x, sizeof(x));
+}
+
+static void bn_exp5(u64 *x)
+{
+    u64 x2[4], x4[4];
+    orc_bn_mont_mul(x2, x, x);
+    orc_bn_mont_mul(x4, x2, x2);
+    orc_bn_mont_mul(x, x4, x);
+}
+
+/* Poseidon over BN254 with t = 5 in the optimised form (full rounds with M,
+ * the last first-half round with P, 60 partial rounds with the sparse S
+ * rows), poseidon/poseidon_bn128.cu:116-192.  The 12 Goldilocks words map to
+ * state[1..4] as element i = w[3i] 2^128 + w[3i+1] 2^64 + w[3i+2]; the
+ * output words of state[0..4) come back the same way (the top limb dropped)
+ * and each is reduced once mod p (:195-204). */
+void orc_poseidon_bn128_permute(u64 *s)
+{
+    const int t = 5, nrp = 60;
+    u64 r2[4], st[5][4], ns[5][4], tmp[4];
+    bn_r2(r2);
+    memset(st, 0, sizeof(st));
+    for (int i = 0; i < 4; i++) {
+        u64 v[4] = {s[3 * i + 2], s[3 * i + 1], s[3 * i], 0};
+        orc_bn_mont_mul(st[i + 1], v, r2);
+    }
+#define ARK(off)                                                                     \
+    for (int k = 0; k < t; k++) orc_bn_add(st[k], st[k], ZK_BN128_C + 4 * ((off) + k))
+#define MIX(mat)                                                                     \
+    do {                                                                             \
+        for (int i = 0; i < t; i++) {                                                \
+            u64 acc[4] = {0, 0, 0, 0};                                               \
+            for (int j = 0; j < t; j++) {                                            \
+                orc_bn_mont_mul(tmp, (mat) + 4 * (t * j + i), st[j]);                \
+                orc_bn_add(acc, acc, tmp);                                           \
+            }                                                                        \
+            memcpy(ns[i], acc, sizeof(acc));                                         \
+        }                                                                            \
+        memcpy(st, ns, sizeof(st));                                                  \
+    } while (0)
+    ARK(0);
+    for (int r = 0; r < 3; r++) {
+        for (int k = 0; k < t; k++) bn_exp5(st[k]);
+        ARK((r + 1) * t);
+        MIX(ZK_BN128_M);
+    }
+    for (int k = 0; k < t; k++) bn_exp5(st[k]);
+    ARK(4 * t);
+    MIX(ZK_BN128_P);
+    for (int r = 0; r < nrp; r++) {
+        bn_exp5(st[0]);
+        orc_bn_add(st[0], st[0], ZK_BN128_C + 4 * (5 * t + r));
+        const u64 *row = ZK_BN128_S + 4 * ((2 * t - 1) * r);
+        u64 n0[4] = {0, 0, 0, 0};
+        for (int j = 0; j < t; j++) {
+            orc_bn_mont_mul(tmp, row + 4 * j, st[j]);
+            orc_bn_add(n0, n0, tmp);
+        }
+        for (int k = 1; k < t; k++) {
+            orc_bn_mont_mul(tmp, st[0], row + 4 * (t + k - 1));
+            orc_bn_add(st[k], st[k], tmp);
+        }
+        memcpy(st[0], n0, sizeof(n0));
+    }
+    for (int r = 0; r < 3; r++) {
+        for (int k = 0; k < t; k++) bn_exp5(st[k]);
+        ARK(5 * t + nrp + r * t);
+        MIX(ZK_BN128_M);
+    }
+    for (int k = 0; k < t; k++) bn_exp5(st[k]);
+    MIX(ZK_BN128_M);
+#undef ARK
+#undef MIX
+    const u64 one[4] = {1, 0, 0, 0};
+    for (int i = 0; i < 4; i++) {
+        u64 v[4];
+        orc_bn_mont_mul(v, st[i], one);
+        const u64 w[3] = {v[2], v[1], v[0]};
+        for (int k = 0; k < 3; k++) s[3 * i + k] = w[k] >= GL_P ? w[k] - GL_P : w[k];
+    }
+}
+
 /* ---------------------------------------------------------------- sponge */
 
-enum { HASH_POSEIDON = 0, HASH_KECCAK = 1, HASH_POSEIDON2 = 3, HASH_MONOLITH = 4 };
+enum { HASH_POSEIDON = 0, HASH_KECCAK = 1, HASH_POSEIDON_BN128 = 2, HASH_POSEIDON2 = 3, HASH_MONOLITH = 4 };
 
 static void permute(u64 *s, int hash_type)
 {
     if (hash_type == HASH_POSEIDON2) orc_poseidon2_permute(s);
     else if (hash_type == HASH_MONOLITH) orc_monolith_permute(s);
+    else if (hash_type == HASH_POSEIDON_BN128) orc_poseidon_bn128_permute(s);
     else orc_poseidon_permute(s);
 }
 

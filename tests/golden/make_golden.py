@@ -36,9 +36,12 @@ MERKLE = [  # (log2 leaves, leaf size, cap height, hash, seed)
     # Keccak-256 (1) and Monolith (4) trees
     (3, 7, 1, 1, 31), (5, 17, 2, 1, 32), (6, 4, 0, 1, 33), (8, 135, 4, 1, 34), (4, 2, 4, 1, 35),
     (3, 9, 1, 4, 41), (6, 12, 3, 4, 42), (8, 135, 4, 4, 43), (5, 4, 5, 4, 44),
+    # Poseidon over BN254 (2)
+    (3, 7, 1, 2, 51), (5, 12, 2, 2, 52), (7, 135, 4, 2, 53), (4, 3, 4, 2, 54),
 ]
 LARGE = [  # hashed outputs: 2^13 x 135, cap height 4, every hash
     (13, 135, 4, 0, 1001), (13, 135, 4, 3, 1002), (13, 135, 4, 1, 1003), (13, 135, 4, 4, 1004),
+    (13, 135, 4, 2, 1005),
 ]
 LDE = [  # (lg_n, rate, cols, coset, seed)
     (4, 1, 3, True, 31), (6, 2, 2, True, 32), (8, 3, 4, False, 33), (10, 1, 5, True, 34),

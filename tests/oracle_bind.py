@@ -31,7 +31,8 @@ def _load_oracle():
         ("orc_gl_add", u64, [u64, u64]), ("orc_gl_sub", u64, [u64, u64]), ("orc_gl_mul", u64, [u64, u64]),
         ("orc_gl_pow", u64, [u64, u64]), ("orc_gl_inv", u64, [u64]), ("orc_gl_omega", u64, [u32]),
         ("orc_poseidon_permute", None, [_u64p]), ("orc_poseidon2_permute", None, [_u64p]),
-        ("orc_monolith_permute", None, [_u64p]),
+        ("orc_monolith_permute", None, [_u64p]), ("orc_poseidon_bn128_permute", None, [_u64p]),
+        ("orc_bn_mont_mul", None, [_u64p, _u64p, _u64p]), ("orc_bn_add", None, [_u64p, _u64p, _u64p]),
         ("orc_keccak", None, [ctypes.c_char_p, u64, ctypes.c_char_p, u32, u32, u32]),
         ("orc_hash_one", None, [_u64p, u64, _u64p, i]), ("orc_hash_two", None, [_u64p, _u64p, _u64p, i]),
         ("orc_fill_digests", i, [_u64p, _u64p, _u64p, u64, u64, u64, u64, u64, i]),
@@ -88,7 +89,8 @@ def hash_two(l, r, hash_type=0):
 
 def permute(state, hash_type=0):
     s = np.ascontiguousarray(state, dtype=np.uint64).copy()
-    fn = {3: orc.orc_poseidon2_permute, 4: orc.orc_monolith_permute}.get(hash_type, orc.orc_poseidon_permute)
+    fn = {2: orc.orc_poseidon_bn128_permute, 3: orc.orc_poseidon2_permute,
+          4: orc.orc_monolith_permute}.get(hash_type, orc.orc_poseidon_permute)
     fn(_ptr(s))
     return s
 

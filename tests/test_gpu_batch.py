@@ -42,7 +42,7 @@ def oracle_commit(coeffs, cols, lg_n, rate, h, ht):
 # (cols, lg_n, rate, cap_height, hash)
 MIX = [(2, 10, 1, 4, 0), (20, 11, 2, 4, 3), (80, 10, 3, 4, 0), (135, 9, 1, 4, 3), (5, 3, 1, 0, 0),
        (7, 2, 2, 4, 0), (3, 1, 1, 2, 3), (4, 5, 2, 7, 0), (9, 12, 1, 2, 3), (1, 6, 1, 1, 0), (135, 13, 1, 4, 0),
-       (17, 9, 2, 4, 1), (34, 8, 1, 3, 4), (3, 7, 1, 2, 1), (12, 6, 3, 4, 4)]
+       (17, 9, 2, 4, 1), (34, 8, 1, 3, 4), (3, 7, 1, 2, 1), (12, 6, 3, 4, 4), (9, 6, 1, 3, 2), (5, 4, 2, 1, 2)]
 
 
 def test_batch_mixed_vs_oracle(zk):
@@ -99,7 +99,7 @@ def test_batch_without_leaf_rows(zk):
 def test_batch_rejects_bad_jobs(zk):
     x = ob.canonical(8 << 4, 1)
     bad = zk.Commit(x, 8, 4, 1, 3, 0)
-    bad.hash_type = 2  # Poseidon-BN128: not provided
+    bad.hash_type = 5  # no such hash
     with pytest.raises(zk.ZeknoxError):
         zk.commit_batch([bad])
     bad.hash_type = 0

@@ -42,6 +42,15 @@ def test_keccak_kat_direct(zk):
         assert [int(v) for v in caps] == want
 
 
+def test_poseidon_bn128_kat_direct(zk):
+    """tests/tests.cu:492-536: Poseidon-BN128 hash_one of 12 words."""
+    inp = np.array([8917524657281059100, 13029010200779371910, 16138660518493481604, 17277322750214136960,
+                    1441151880423231822, 0, 0, 0, 0, 0, 0, 0], np.uint64)
+    _, caps = gpu_tree(zk, inp, 12, 0, 2)
+    assert [int(v) for v in caps] == [2163910501769503938, 9976732063159483418, 662985512748194034,
+                                      3626198389901409849]
+
+
 def test_monolith_kat_direct(zk):
     """tests/tests.cu:201-235: Monolith hash_one(0..11)."""
     _, caps = gpu_tree(zk, np.arange(12, dtype=np.uint64), 12, 0, 4)
@@ -60,7 +69,7 @@ SHAPES = [(1, 7, 0), (1, 7, 1), (2, 9, 1), (3, 5, 1), (4, 2, 2), (5, 4, 3), (6, 
           (8, 17, 8), (9, 1, 4), (10, 135, 4), (11, 20, 0), (12, 80, 4), (9, 12, 9), (10, 400, 2)]
 
 
-@pytest.mark.parametrize("ht", [0, 1, 3, 4])
+@pytest.mark.parametrize("ht", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("lg_l,ls,h", SHAPES)
 def test_tree_vs_oracle(zk, ht, lg_l, ls, h):
     print(f"tree lg_l={lg_l} ls={ls} h={h} ht={ht}", flush=True)
@@ -106,5 +115,5 @@ def test_invalid_arguments_are_reported(zk):
     with pytest.raises(zk.ZeknoxError):  # leaves not a power of two
         zk.fill_digests_buf_linear_gpu_with_gpu_ptr(d, c, l, 2 * (24 - 4), 4, 24, 8, 2, 0)
     with pytest.raises(zk.ZeknoxError):  # hash type not provided
-        zk.fill_digests_buf_linear_gpu_with_gpu_ptr(d, c, l, 2 * (32 - 4), 4, 32, 8, 2, 2)
+        zk.fill_digests_buf_linear_gpu_with_gpu_ptr(d, c, l, 2 * (32 - 4), 4, 32, 8, 2, 5)
     assert np.all(c.to_host() == 0)  # nothing written

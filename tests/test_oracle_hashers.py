@@ -117,3 +117,68 @@ def test_merkle_matches_reference_cpu(ht, lg, ls, h):
     dig, cap = ob.merkle(leaves, ls, h, ht)
     assert np.array_equal(cap, c)
     assert np.array_equal(dig, d[:nd * 4])
+
+
+# ---- Poseidon over BN254 (hash type 2) -------------------------------------------
+
+BN_R = 21888242871839275222246405745257275088548364400416034343698204186575808495617
+
+
+def _limbs(x):
+    return np.array([(x >> (64 * i)) & (2**64 - 1) for i in range(4)], np.uint64)
+
+
+def _int(a):
+    return sum(int(v) << (64 * i) for i, v in enumerate(a))
+
+
+def test_bn254_montgomery_arithmetic_vs_big_integers():
+    import random
+    rnd = random.Random(7)
+    rinv = pow(2**256, -1, BN_R)
+    for _ in range(200):
+        a, b = rnd.randrange(BN_R), rnd.randrange(BN_R)
+        z = np.zeros(4, np.uint64)
+        ob.orc.orc_bn_mont_mul(ob._ptr(z), ob._ptr(_limbs(a)), ob._ptr(_limbs(b)))
+        assert _int(z) == a * b * rinv % BN_R
+        ob.orc.orc_bn_add(ob._ptr(z), ob._ptr(_limbs(a)), ob._ptr(_limbs(b)))
+        assert _int(z) == (a + b) % BN_R
+    for a, b in [(BN_R - 1, BN_R - 1), (0, BN_R - 1), (1, 1)]:
+        z = np.zeros(4, np.uint64)
+        ob.orc.orc_bn_mont_mul(ob._ptr(z), ob._ptr(_limbs(a)), ob._ptr(_limbs(b)))
+        assert _int(z) == a * b * rinv % BN_R
+
+
+def test_poseidon_bn128_reference_kat():
+    """tests/tests.cu:492-536: hash_one of 12 words."""
+    inp = np.array([8917524657281059100, 13029010200779371910, 16138660518493481604, 17277322750214136960,
+                    1441151880423231822, 0, 0, 0, 0, 0, 0, 0], np.uint64)
+    want = [2163910501769503938, 9976732063159483418, 662985512748194034, 3626198389901409849]
+    assert [int(v) for v in ob.hash_one(inp, 2)] == want
+
+
+@needs_ref
+@pytest.mark.parametrize("n", [1, 4, 5, 8, 12, 17, 135])
+def test_poseidon_bn128_hash_one_matches_reference_cpu(n):
+    x = ob.canonical(n, 300 + n)
+    want = np.zeros(4, np.uint64)
+    REF.ref_hash_one(ob._ptr(x), n, ob._ptr(want), 2)
+    assert np.array_equal(ob.hash_one(x, 2), want)
+
+
+@needs_ref
+def test_poseidon_bn128_hash_two_and_tree_match_reference_cpu():
+    for seed in range(3):
+        l, r = ob.canonical(4, 40 + seed), ob.canonical(4, 50 + seed)
+        want = np.zeros(4, np.uint64)
+        REF.ref_hash_two(ob._ptr(l), ob._ptr(r), ob._ptr(want), 2)
+        assert np.array_equal(ob.hash_two(l, r, 2), want)
+    for lg, ls, h in [(3, 7, 1), (5, 12, 2), (4, 3, 4)]:
+        leaves = ob.canonical(ls << lg, 3 * lg + ls)
+        nl, nc = 1 << lg, 1 << h
+        nd = 2 * (nl - nc) if nl != nc else 0
+        d = np.zeros(max(nd, 1) * 4, np.uint64)
+        c = np.zeros(nc * 4, np.uint64)
+        REF.fill_digests_buf_linear_cpu(ob._ptr(d), ob._ptr(c), ob._ptr(leaves), nd, nc, nl, ls, h, 2)
+        dig, cap = ob.merkle(leaves, ls, h, 2)
+        assert np.array_equal(cap, c) and np.array_equal(dig, d[:nd * 4])

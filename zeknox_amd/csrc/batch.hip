@@ -22,8 +22,9 @@ uint64_t lde_elems(const CommitJob &j) { return (uint64_t)j.cols << (j.lg_n + j.
 
 void check_commit_job(const CommitJob &j)
 {
-    if (j.hash_type != kPoseidon && j.hash_type != kKeccak && j.hash_type != kPoseidon2 && j.hash_type != kMonolith)
-        invalid("commit job: hash type must be 0 (Poseidon), 1 (Keccak), 3 (Poseidon2) or 4 (Monolith)");
+    if (j.hash_type > kMonolith)
+        invalid("commit job: hash type must be 0 (Poseidon), 1 (Keccak), 2 (Poseidon-BN128), 3 (Poseidon2) or "
+                "4 (Monolith)");
     if (j.cols == 0) invalid("commit job: no columns");
     if (j.lg_n == 0 || j.lg_n + j.rate > 30) invalid("commit job: log degree out of range");
     if (j.rate < 1 || j.rate > kMaxRateBits) invalid("commit job: rate_bits must be in [1, 5]");
@@ -188,7 +189,7 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
             host.resize(align_up(host.size()));
         }
     for (size_t k = 0; k < max_mk; k++)
-        for (int ht : {kPoseidon, kKeccak, kPoseidon2, kMonolith})
+        for (int ht : {kPoseidon, kKeccak, kPoseidonBN128, kPoseidon2, kMonolith})
             for (uint32_t tl = kMerkleMinTileLog; tl <= (uint32_t)kMerkleMaxTileLog; tl++) {
                 std::vector<MerkleArgs> list;
                 uint64_t blocks = 0;

@@ -9,6 +9,7 @@
 //                 as the fast-partial-round form of poseidon/poseidon.cpp).
 // Poseidon2     : poseidon2/poseidon2.cu:65-209.
 // Monolith      : monolith/monolith.cu:104-227 (monolith.h).
+// Poseidon-BN128: poseidon/poseidon_bn128.cu:116-204 (poseidon_bn128.h).
 // Keccak-256    : keccak/keccak.cu (keccak.h; its own sponge).
 // Sponge        : poseidon/poseidon_permutation.hpp:83-139 /
 //                 poseidon_permutation.cuh:83-134.
@@ -18,6 +19,7 @@
 #include "keccak.h"
 #include "merkle_types.h"
 #include "monolith.h"
+#include "poseidon_bn128.h"
 
 namespace zkx {
 
@@ -134,8 +136,8 @@ __device__ __forceinline__ void p2_external(uint64_t s[12])
         }
 }
 
-// Canonical sum of the 12 state elements (each below 2^64): 32-bit carry
-// chain into three words, one reduction.
+// Sum of the 12 state elements (each below 2^64), lazy: 32-bit carry chain
+// into three words, one reduction.
 __device__ __forceinline__ uint64_t sum12(const uint64_t s[12])
 {
     uint32_t lo = 0, hi = 0, top = 0;
@@ -146,11 +148,11 @@ __device__ __forceinline__ uint64_t sum12(const uint64_t s[12])
         hi = __builtin_addc(hi, gl::hi32(s[i]), c1, &c2);
         top = __builtin_addc(top, 0u, c2, &c3);
     }
-    return gl::reduce96(gl::pack(lo, hi), top);
+    return gl::reduce96_lazy(gl::pack(lo, hi), top);
 }
 
 // Internal layer: s_i <- s_i * d_i + sum(s) (d = MATRIX_DIAG_12 = diag - 1),
-// each as one 128-bit value reduced once (lazy inputs, canonical outputs).
+// each as one 128-bit value reduced once (lazy in and out).
 __device__ __forceinline__ void p2_internal(uint64_t s[12])
 {
     const uint64_t sum = sum12(s);
@@ -163,7 +165,7 @@ __device__ __forceinline__ void p2_internal(uint64_t s[12])
         const uint32_t l1 = __builtin_addc(gl::hi32(lo), gl::hi32(sum), c1, &c2);
         const uint32_t h0 = __builtin_addc(gl::lo32(hi), 0u, c2, &c3);
         const uint32_t h1 = __builtin_addc(gl::hi32(hi), 0u, c3, &c4);
-        s[i] = gl::reduce128(gl::pack(l0, l1), gl::pack(h0, h1));
+        s[i] = gl::reduce128_lazy(gl::pack(l0, l1), gl::pack(h0, h1));
     }
 }
 
@@ -191,6 +193,7 @@ __device__ __forceinline__ void permute(uint64_t s[12])
 {
     if constexpr (HT == kPoseidon2) poseidon2_permute(s);
     else if constexpr (HT == kMonolith) monolith_permute(s);
+    else if constexpr (HT == kPoseidonBN128) poseidon_bn128_permute(s);
     else poseidon_permute(s);
 }
 

@@ -19,9 +19,9 @@ uint32_t log2u(uint64_t x) { return 63 - __builtin_clzll(x); }
 
 void check_merkle(const MerkleSpec &s)
 {
-    if (s.hash_type != kPoseidon && s.hash_type != kKeccak && s.hash_type != kPoseidon2 && s.hash_type != kMonolith)
+    if (s.hash_type > kMonolith)
         invalid("hash type " + std::to_string(s.hash_type) +
-                " is not provided (Poseidon = 0, Keccak = 1, Poseidon2 = 3, Monolith = 4)");
+                " is not provided (Poseidon 0, Keccak 1, Poseidon-BN128 2, Poseidon2 3, Monolith 4)");
     if (!s.caps || !s.leaves) invalid("null cap or leaf buffer");
     if (!is_pow2(s.num_leaves) || !is_pow2(s.num_caps) || s.num_caps > s.num_leaves)
         invalid("leaves and caps must be powers of two with caps <= leaves");
@@ -77,6 +77,9 @@ void launch_merkle_pass(hipStream_t stream, int hash_type, bool from_leaves, con
     case kPoseidon2: launch_merkle_hash<kPoseidon2>(stream, from_leaves, dev_list, n, one, blocks, tile_log_max); break;
     case kKeccak: launch_merkle_hash<kKeccak>(stream, from_leaves, dev_list, n, one, blocks, tile_log_max); break;
     case kMonolith: launch_merkle_hash<kMonolith>(stream, from_leaves, dev_list, n, one, blocks, tile_log_max); break;
+    case kPoseidonBN128:
+        launch_merkle_hash<kPoseidonBN128>(stream, from_leaves, dev_list, n, one, blocks, tile_log_max);
+        break;
     default: launch_merkle_hash<kPoseidon>(stream, from_leaves, dev_list, n, one, blocks, tile_log_max); break;
     }
     ZKX_HIP(hipGetLastError());

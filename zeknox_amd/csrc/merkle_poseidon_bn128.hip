@@ -1,0 +1,9 @@
+// Merkle kernels hashing with Poseidon over BN254 (hash type of merkle/merkle.h).
+#include "merkle_launch_impl.h"
+
+namespace zkx {
+
+template void launch_merkle_hash<kPoseidonBN128>(hipStream_t, bool, const MerkleArgs *, uint32_t, const MerkleArgs &,
+                                                 uint64_t, uint32_t);
+
+}  // namespace zkx

@@ -5,8 +5,8 @@
 
 namespace zkx {
 
-// merkle/merkle.h:21-28 hash type numbers (2, Poseidon-BN128, is not provided)
-enum HashKind : int { kPoseidon = 0, kKeccak = 1, kPoseidon2 = 3, kMonolith = 4 };
+// merkle/merkle.h:21-28 hash type numbers
+enum HashKind : int { kPoseidon = 0, kKeccak = 1, kPoseidonBN128 = 2, kPoseidon2 = 3, kMonolith = 4 };
 
 // Work-group sizes of the Merkle kernels: 2^8, 2^9 or 2^10 lanes, one leaf
 // (or node) per lane.
