@@ -60,10 +60,10 @@ typedef struct {
 /* ---- native/merkle/merkle.h:21-28 ------------------------------------- */
 enum HashType {
     HashPoseidon = 0,
-    HashKeccak = 1,        /* not provided by this build: rejected */
-    HashPoseidonBN128 = 2, /* not provided by this build: rejected */
+    HashKeccak = 1,        /* Keccak-256, 25-byte digests */
+    HashPoseidonBN128 = 2, /* Poseidon over BN254, t = 5 */
     HashPoseidon2 = 3,
-    HashMonolith = 4       /* not provided by this build: rejected */
+    HashMonolith = 4
 };
 
 /* ---- native/lib.h:21-51, native/lib.cu:36-175 ------------------------- */

@@ -39,7 +39,8 @@ void *zkx_stream(size_t device_id);
  * 2^lg_n coefficients (column-major, device) with blowup 2^rate; the leaf
  * rows leaves[t * cols + b] = LDE_b[bitrev(t)] (compute_transpose_rev of the
  * LDE; optional, NULL to skip); the Merkle forest of those rows with
- * 2^cap_height caps (hash_type 0 Poseidon, 3 Poseidon2).  Outputs are
+ * 2^cap_height caps (hash_type as merkle.h: 0 Poseidon, 1 Keccak-256,
+ * 2 Poseidon-BN128, 3 Poseidon2, 4 Monolith).  Outputs are
  * bit-identical to compute_batched_lde -> compute_transpose_rev ->
  * fill_digests_buf_linear_gpu_with_gpu_ptr on the same inputs.  All pointers
  * are device pointers; sizes as in merkle.h. */

@@ -20,7 +20,10 @@ LIB_PATH = os.path.join(_HERE, "libzeknox.so")
 
 P = 0xFFFFFFFF00000001
 HASH_POSEIDON = 0
+HASH_KECCAK = 1
+HASH_POSEIDON_BN128 = 2
 HASH_POSEIDON2 = 3
+HASH_MONOLITH = 4
 FORWARD, INVERSE = 0, 1
 
 
