@@ -117,3 +117,16 @@ def test_invalid_arguments_are_reported(zk):
     with pytest.raises(zk.ZeknoxError):  # hash type not provided
         zk.fill_digests_buf_linear_gpu_with_gpu_ptr(d, c, l, 2 * (32 - 4), 4, 32, 8, 2, 5)
     assert np.all(c.to_host() == 0)  # nothing written
+
+
+def test_four_pass_tree_in_one_launch(zk):
+    """2^25 leaves, one cap, 256-leaf tiles: the leaf pass and three upper
+    passes run in one launch (the last work-group of each group of tiles goes
+    on).  Keccak with 2-element leaves keeps the oracle's work to the 2^25
+    inner nodes."""
+    print("four-pass tree 2^25 x 2, cap height 0, Keccak", flush=True)
+    leaves = ob.canonical(2 << 25, 2525)
+    dig, caps = gpu_tree(zk, leaves, 2, 0, 1)
+    edig, ecap = ob.merkle(leaves, 2, 0, 1)
+    assert np.array_equal(caps, ecap)
+    assert np.array_equal(dig, edig)

@@ -9,9 +9,10 @@
 //
 // An Engine runs any number of uneven jobs together on one stream: one
 // launch per NTT pass index covers that pass of every job, one launch per
-// Merkle pass index and hash kind covers that pass of every tree, so small
-// jobs share the GPU and the latency-bound top levels of different trees run
-// side by side.  The LDE stays in HBM in its natural DIF output order
+// hash kind and work-group size builds every tree (the upper levels of each
+// tree are finished inside the same launch by the last work-group of each
+// group of tiles), so small jobs share the GPU and the latency-bound top
+// levels of different trees overlap with other trees' leaf hashing.  The LDE stays in HBM in its natural DIF output order
 // (bit-reversed, column-major) and the leaf kernels read it there directly:
 // no bit-reversal or transpose pass; the leaf rows are written by the leaf
 // kernel as it reads them.
@@ -66,8 +67,12 @@ private:
     Device &dev_;
     hipStream_t stream_ = nullptr;
     bool own_stream_ = false;
+    hipStream_t stream2_ = nullptr;     // second half of a batch (default priority)
+    hipEvent_t ev_ntt0_ = nullptr;      // first half's NTT passes enqueued before this
+    hipEvent_t ev_done1_ = nullptr;     // second half complete
     DevBuf scratch_;           // LDE of every job, DIF order
     DevBuf desc_;              // descriptor lists (device)
+    DevBuf ctr_;               // arrival counters of the fused Merkle passes (zero between launches)
     void *pinned_ = nullptr;   // descriptor staging (host, pinned)
     size_t pinned_bytes_ = 0;
     hipEvent_t staged_ = nullptr;  // last staging copy

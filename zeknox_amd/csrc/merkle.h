@@ -23,11 +23,16 @@ void check_merkle(const MerkleSpec &s);
 // Pass descriptors of one tree, in launch order (tile_begin left 0), with
 // tiles of at most 2^tile_log_max nodes (8..10).
 void plan_merkle(const MerkleSpec &s, std::vector<MerkleArgs> &passes, uint32_t tile_log_max);
+// One descriptor that runs every pass of the tree in one launch (the upper
+// passes fused into the first); returns the arrival counters it needs
+// (its `ctr` is left null for the caller to point at that many zeroed words).
+uint64_t fuse_merkle_passes(const std::vector<MerkleArgs> &passes, MerkleArgs &out);
 // One launch over `n` job descriptors in device memory (n > 0), or over the
 // single descriptor `one` (n == 0); `blocks` work-groups of 2^tile_log_max
 // lanes in total (every descriptor's tile_log <= tile_log_max).
 void launch_merkle_pass(hipStream_t stream, int hash_type, bool from_leaves, const MerkleArgs *dev_list,
                         uint32_t n, const MerkleArgs &one, uint64_t blocks, uint32_t tile_log_max);
+// The whole tree in one launch on the device stream; call with d.mu held.
 void merkle_build(Device &d, const MerkleSpec &s);
 
 }  // namespace zkx

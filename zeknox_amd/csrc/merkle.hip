@@ -65,6 +65,22 @@ void plan_merkle(const MerkleSpec &s, std::vector<MerkleArgs> &passes, uint32_t 
     }
 }
 
+uint64_t fuse_merkle_passes(const std::vector<MerkleArgs> &passes, MerkleArgs &out)
+{
+    if (passes.empty() || passes.size() > (size_t)kMerkleMaxPasses) invalid("internal: Merkle pass plan");
+    out = passes[0];
+    out.n_next = (uint32_t)passes.size() - 1;
+    out.ctr = nullptr;
+    uint64_t ctrs = 0;
+    for (size_t p = 1; p < passes.size(); p++) {
+        out.next_tile_log[p - 1] = passes[p].tile_log;
+        out.next_ctr[p - 1] = (uint32_t)ctrs;
+        ctrs += passes[p].count >> passes[p].tile_log;
+    }
+    if (ctrs > 0xFFFFFFFFull) invalid("tree too large");
+    return ctrs;
+}
+
 void launch_merkle_pass(hipStream_t stream, int hash_type, bool from_leaves, const MerkleArgs *dev_list,
                         uint32_t n, const MerkleArgs &one, uint64_t blocks, uint32_t tile_log_max)
 {
@@ -91,12 +107,18 @@ void merkle_build(Device &d, const MerkleSpec &s)
     const uint32_t tl = d.tuning_default().merkle_tile_log;
     std::vector<MerkleArgs> passes;
     plan_merkle(s, passes, tl);
-    bool first = true;
-    for (MerkleArgs &p : passes) {
-        p.tile_begin = 0;
-        launch_merkle_pass(d.stream, (int)s.hash_type, first, nullptr, 0, p, p.count >> p.tile_log, tl);
-        first = false;
+    MerkleArgs f;
+    const uint64_t nctr = fuse_merkle_passes(passes, f);
+    f.tile_begin = 0;
+    if (!nctr) {
+        launch_merkle_pass(d.stream, (int)s.hash_type, true, nullptr, 0, f, f.count >> f.tile_log, tl);
+        return;
     }
+    // fused upper passes read their fields from a descriptor in device memory
+    f.ctr = d.merkle_counters(nctr);
+    MerkleArgs *dj = d.merkle_descriptor();
+    ZKX_HIP(hipMemcpyAsync(dj, &f, sizeof(f), hipMemcpyHostToDevice, d.stream));
+    launch_merkle_pass(d.stream, (int)s.hash_type, true, dj, 1, MerkleArgs{}, f.count >> f.tile_log, tl);
 }
 
 }  // namespace zkx

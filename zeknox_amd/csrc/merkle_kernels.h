@@ -22,6 +22,14 @@
 // of a wave read consecutive words of each column, and optionally writes the
 // leaf rows (the transpose) on the way.
 //
+// Upper passes in the same launch.  A tile that stops at its root below the
+// subtree top counts itself in an arrival counter of its group (the 2^tl
+// tiles whose roots form one tile of the next level); the work-group that
+// arrives last, after the others' roots are visible, loads the group's roots
+// and reduces that tile too, and so on up to the caps.  Nothing waits: every
+// work-group either continues or exits.  The last arrival resets its counter,
+// so counters are zero between launches.
+//
 // A launch covers a list of independent jobs (trees or tree passes);
 // work-group b runs tile b - tile_begin of the job with the largest
 // tile_begin <= b.
@@ -53,7 +61,10 @@ __global__ void __launch_bounds__(TPB) k_merkle_tile(const MerkleArgs *__restric
                                                    MerkleArgs one)
 {
     __shared__ uint64_t lds[TPB][4];
-    const MerkleArgs a = njobs ? *find_job(jobs, njobs) : one;
+    __shared__ uint32_t lds_last;
+    const MerkleArgs *job = njobs ? find_job(jobs, njobs) : nullptr;
+    const MerkleArgs a = njobs ? *job : one;
+    uint32_t up = 0;  // upper passes done
     const uint32_t t = threadIdx.x;
     uint32_t cur = 1u << a.tile_log;  // nodes of this tile at the current level
     uint32_t lg_w = a.lg_w;           // log2 per-subtree width of the current level
@@ -89,8 +100,46 @@ __global__ void __launch_bounds__(TPB) k_merkle_tile(const MerkleArgs *__restric
     __syncthreads();
 
     while (true) {
+        if (cur == 1) {
+            // tile root (at any level, the width-2 one included): continue
+            // with the group's tile if this work-group is the last of its
+            // group to get here
+            // The upper-pass fields are read here, from the descriptor list
+            // (fused launches always have one), rather than kept in
+            // registers through the hashing.
+            if (!job) return;
+            const volatile MerkleArgs *vj = job;
+            if (up >= vj->n_next) return;
+            const uint32_t tl = vj->next_tile_log[up];
+            const uint32_t c0 = vj->next_ctr[up];
+            uint32_t *const ctr = vj->ctr;
+            const uint64_t grp = first >> tl;
+            __threadfence();  // this tile's digests, before the arrival
+            __syncthreads();
+            if (t == 0) {
+                uint32_t *c = ctr + c0 + grp;
+                const uint32_t arrived = atomicAdd(c, 1u);
+                const bool last = arrived == (1u << tl) - 1;
+                if (last) atomicExch(c, 0u);
+                lds_last = last ? 1u : 0u;
+            }
+            __syncthreads();
+            if (!lds_last) return;
+            __threadfence();  // the other tiles' roots, after their arrivals
+            up++;
+            cur = 1u << tl;
+            first = grp << tl;
+            if (t < cur) {
+                const uint64_t g = first + t;
+                const uint64_t *src = node_ptr(a, g >> lg_w, lg_w, g & ((1ull << lg_w) - 1));
+#pragma unroll
+                for (int i = 0; i < 4; i++) lds[t][i] = src[i];
+            }
+            __syncthreads();
+            continue;
+        }
         if (lg_w == 1) {
-            // width-2 top level: caps of the subtrees in this tile
+            // width-2 top level (cur >= 2 here): caps of the subtrees in this tile
             const uint32_t pairs = cur >> 1;
             if (t < pairs) {
                 uint64_t out[4];
@@ -101,7 +150,6 @@ __global__ void __launch_bounds__(TPB) k_merkle_tile(const MerkleArgs *__restric
             }
             return;
         }
-        if (cur == 1) return;  // tile root: next pass continues from here
         const uint32_t half = cur >> 1;
         uint64_t out[4];
         if (t < half) hash_two<HT>(lds[2 * t], lds[2 * t + 1], out);

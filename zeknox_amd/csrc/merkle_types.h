@@ -12,6 +12,8 @@ enum HashKind : int { kPoseidon = 0, kKeccak = 1, kPoseidonBN128 = 2, kPoseidon2
 // (or node) per lane.
 constexpr int kMerkleMinTileLog = 8;
 constexpr int kMerkleMaxTileLog = 10;
+// Passes of one tree: tiles of >= 2^8 nodes reduce 2^32 leaves in 4.
+constexpr int kMerkleMaxPasses = 4;
 
 struct MerkleArgs {
     const uint64_t *leaves;  // first pass: leaf source
@@ -26,7 +28,12 @@ struct MerkleArgs {
     uint32_t lg_w;           // log2 width of the input level per subtree (0: caps == leaves)
     uint32_t tile_log;       // log2 nodes per work-group tile
     uint32_t leaf_size;
-    uint32_t pad_;
+    // Upper passes run inside the same launch: the last work-group to finish
+    // a group of 2^next_tile_log[p] tiles goes on with the tile of their roots.
+    uint32_t n_next;
+    uint32_t next_tile_log[kMerkleMaxPasses - 1];
+    uint32_t next_ctr[kMerkleMaxPasses - 1];  // first arrival counter of each upper pass
+    uint32_t *ctr;           // arrival counters, zero before and after the launch (null: n_next == 0)
 };
 
 }  // namespace zkx

@@ -19,6 +19,8 @@
 
 namespace zkx {
 
+struct MerkleArgs;
+
 // Thrown inside the library, converted to RustError{code, message} at the
 // C boundary.  code 1 = invalid argument; -e = HIP error e.
 struct Error : std::runtime_error {
@@ -72,6 +74,15 @@ struct Device {
     uint64_t coset_gen = 0;
     // LDE factor tables keyed by (lg_n, rate, inverse, generator)
     std::map<std::tuple<uint32_t, uint32_t, bool, uint64_t>, uint64_t *> lde_factors;
+
+    // Zeroed arrival counters of the fused Merkle launches on the device
+    // stream (each launch leaves them zero); grows on demand, under mu.
+    DevBuf merkle_ctr;
+    uint32_t *merkle_counters(uint64_t n);
+    // One device-side descriptor slot for those launches (written with a
+    // stream-ordered copy before each launch), under mu.
+    DevBuf merkle_desc;
+    struct MerkleArgs *merkle_descriptor();
 
     const uint64_t *twiddle_table(uint32_t lg);  // builds on first use
     void build_coset(uint32_t lg, uint64_t gen);

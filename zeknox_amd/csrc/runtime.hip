@@ -4,8 +4,10 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <memory>
 
+#include "merkle_types.h"
 #include "ntt.h"
 
 namespace zkx {
@@ -72,6 +74,23 @@ Device &device(size_t id)
 }
 
 void Device::sync() { ZKX_HIP(hipStreamSynchronize(stream)); }
+
+uint32_t *Device::merkle_counters(uint64_t n)
+{
+    const size_t bytes = n * sizeof(uint32_t);
+    if (merkle_ctr.bytes < bytes) {
+        sync();  // no launch may still use the old counters
+        merkle_ctr = DevBuf(std::max<size_t>(bytes, 4096));
+        ZKX_HIP(hipMemsetAsync(merkle_ctr.p, 0, merkle_ctr.bytes, stream));
+    }
+    return merkle_ctr.as<uint32_t>();
+}
+
+MerkleArgs *Device::merkle_descriptor()
+{
+    if (!merkle_desc.p) merkle_desc = DevBuf(sizeof(MerkleArgs));
+    return merkle_desc.as<MerkleArgs>();
+}
 
 bool valid_tuning(const Tuning &t)
 {
