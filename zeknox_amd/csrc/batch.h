@@ -67,9 +67,6 @@ private:
     Device &dev_;
     hipStream_t stream_ = nullptr;
     bool own_stream_ = false;
-    hipStream_t stream2_ = nullptr;     // second half of a batch (default priority)
-    hipEvent_t ev_ntt0_ = nullptr;      // first half's NTT passes enqueued before this
-    hipEvent_t ev_done1_ = nullptr;     // second half complete
     DevBuf scratch_;           // LDE of every job, DIF order
     DevBuf desc_;              // descriptor lists (device)
     DevBuf ctr_;               // arrival counters of the fused Merkle passes (zero between launches)

@@ -47,21 +47,14 @@ Engine::Engine(Device &d, hipStream_t stream) : dev_(d), stream_(stream)
         ZKX_HIP(hipStreamCreate(&stream_));
         own_stream_ = true;
     }
-    ZKX_HIP(hipStreamCreate(&stream2_));
     ZKX_HIP(hipEventCreateWithFlags(&staged_, hipEventDisableTiming));
-    ZKX_HIP(hipEventCreateWithFlags(&ev_ntt0_, hipEventDisableTiming));
-    ZKX_HIP(hipEventCreateWithFlags(&ev_done1_, hipEventDisableTiming));
 }
 
 Engine::~Engine()
 {
     (void)hipSetDevice(dev_.id);
     if (stream_) (void)hipStreamSynchronize(stream_);
-    if (stream2_) (void)hipStreamSynchronize(stream2_);
     if (staged_) (void)hipEventDestroy(staged_);
-    if (ev_ntt0_) (void)hipEventDestroy(ev_ntt0_);
-    if (ev_done1_) (void)hipEventDestroy(ev_done1_);
-    if (stream2_) (void)hipStreamDestroy(stream2_);
     if (pinned_) (void)hipHostFree(pinned_);
     if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
 }
@@ -164,32 +157,15 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
         max_ntt = std::max(max_ntt, ntt_plan[i].size());
     }
 
-    // ---- two halves of the batch (by work, in job order) when there are two
-    // or more jobs: the second half's NTT passes (memory- and latency-bound
-    // in part) run beside the first half's leaf hashing (VALU-bound)
-    std::vector<int> half(n, 0);
-    bool split = false;
-    if (n >= 2) {
-        double total = 0, acc = 0;
-        for (size_t i = 0; i < n; i++) total += (double)lde_elems(jobs[i]);
-        for (size_t i = 0; i < n; i++) {
-            half[i] = acc >= total / 2 ? 1 : 0;
-            acc += (double)lde_elems(jobs[i]);
-        }
-        split = half[n - 1] == 1;
-    }
-
-    // ---- descriptor lists, per half: NTT pass k of every job; every tree of
-    // each hash kind and work-group size.  All staged with one host->device copy.
+    // ---- descriptor lists: NTT pass k of every job; every tree of each hash
+    // kind and work-group size.  All staged with one host->device copy.
     struct Launch {
         size_t offset;  // bytes into desc_
         uint32_t count;
         uint64_t blocks;
         int kind;       // 0 NTT, 1 Merkle
         int hash;       // NTT: contiguous; Merkle: hash type
-        bool leaves;
         uint32_t tile_log;  // Merkle work-group size (log2 lanes)
-        int half;
     };
     std::vector<Launch> launches;
     std::vector<char> host;
@@ -205,77 +181,56 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
         ctr_ = DevBuf(std::max<size_t>(total_ctr * sizeof(uint32_t), 4096));
         ZKX_HIP(hipMemsetAsync(ctr_.p, 0, ctr_.bytes, stream_));
     }
-    for (int h = 0; h <= (split ? 1 : 0); h++) {
-        for (size_t k = 0; k < max_ntt; k++)
-            for (int contig = 0; contig < 2; contig++) {
-                std::vector<NttPass> list;
-                uint64_t blocks = 0;
-                for (size_t i = 0; i < n; i++) {
-                    if (half[i] != h || k >= ntt_plan[i].size() || (ntt_plan[i][k].s_lo == 0) != (contig == 1))
-                        continue;
-                    NttPass p = ntt_plan[i][k];
-                    p.tile_begin = blocks;
-                    blocks += ntt_tiles[i];
-                    list.push_back(p);
-                }
-                if (list.empty()) continue;
-                const size_t off = push(list.data(), list.size() * sizeof(NttPass));
-                launches.push_back({off, (uint32_t)list.size(), blocks, 0, contig, false, 0, h});
-                host.resize(align_up(host.size()));
+    for (size_t k = 0; k < max_ntt; k++)
+        for (int contig = 0; contig < 2; contig++) {
+            std::vector<NttPass> list;
+            uint64_t blocks = 0;
+            for (size_t i = 0; i < n; i++) {
+                if (k >= ntt_plan[i].size() || (ntt_plan[i][k].s_lo == 0) != (contig == 1)) continue;
+                NttPass p = ntt_plan[i][k];
+                p.tile_begin = blocks;
+                blocks += ntt_tiles[i];
+                list.push_back(p);
             }
-        for (int ht : {kPoseidon, kKeccak, kPoseidonBN128, kPoseidon2, kMonolith})
-            for (uint32_t tl = kMerkleMinTileLog; tl <= (uint32_t)kMerkleMaxTileLog; tl++) {
-                std::vector<MerkleArgs> list;
-                uint64_t blocks = 0;
-                for (size_t i = 0; i < n; i++) {
-                    if (half[i] != h || (int)jobs[i].hash_type != ht || tun[i].merkle_tile_log != tl) continue;
-                    MerkleArgs a = mk[i];
-                    a.tile_begin = blocks;
-                    a.ctr = ctr_.as<uint32_t>() + mk_ctr[i];
-                    blocks += a.count >> a.tile_log;
-                    list.push_back(a);
-                }
-                if (list.empty()) continue;
-                const size_t off = push(list.data(), list.size() * sizeof(MerkleArgs));
-                launches.push_back({off, (uint32_t)list.size(), blocks, 1, ht, true, tl, h});
-                host.resize(align_up(host.size()));
+            if (list.empty()) continue;
+            const size_t off = push(list.data(), list.size() * sizeof(NttPass));
+            launches.push_back({off, (uint32_t)list.size(), blocks, 0, contig, 0});
+            host.resize(align_up(host.size()));
+        }
+    for (int ht : {kPoseidon, kKeccak, kPoseidonBN128, kPoseidon2, kMonolith})
+        for (uint32_t tl = kMerkleMinTileLog; tl <= (uint32_t)kMerkleMaxTileLog; tl++) {
+            std::vector<MerkleArgs> list;
+            uint64_t blocks = 0;
+            for (size_t i = 0; i < n; i++) {
+                if ((int)jobs[i].hash_type != ht || tun[i].merkle_tile_log != tl) continue;
+                MerkleArgs a = mk[i];
+                a.tile_begin = blocks;
+                a.ctr = ctr_.as<uint32_t>() + mk_ctr[i];
+                blocks += a.count >> a.tile_log;
+                list.push_back(a);
             }
-    }
+            if (list.empty()) continue;
+            const size_t off = push(list.data(), list.size() * sizeof(MerkleArgs));
+            launches.push_back({off, (uint32_t)list.size(), blocks, 1, ht, tl});
+            host.resize(align_up(host.size()));
+        }
     size_t off0 = 0;
     stage(host.data(), host.size(), off0);
     (void)off0;
 
-    // ---- launches: half 0 on the engine stream; half 1 on the second
-    // stream after half 0's NTT passes; the engine stream then waits for
-    // half 1, so its completion is the batch's
-    bool ntt0_marked = false;
+    // ---- launches, in dependency order on one stream (running the second
+    // half of a batch's NTT passes beside the first half's hashing on a
+    // second stream was measured 7% slower: the co-resident NTT work-groups
+    // take the hash kernel's occupancy)
     for (const Launch &l : launches) {
         const char *base = static_cast<const char *>(desc_.p) + l.offset;
-        hipStream_t s = stream_;
-        if (l.half == 1) {
-            s = stream2_;
-            if (!ntt0_marked) {
-                ZKX_HIP(hipEventRecord(ev_ntt0_, stream_));
-                ZKX_HIP(hipStreamWaitEvent(stream2_, ev_ntt0_, 0));
-                ntt0_marked = true;
-            }
-        }
         if (l.kind == 0) {
-            launch_ntt_list(s, reinterpret_cast<const NttPass *>(base), l.count, l.blocks, l.hash == 1, "batch_ntt");
+            launch_ntt_list(stream_, reinterpret_cast<const NttPass *>(base), l.count, l.blocks, l.hash == 1,
+                            "batch_ntt");
         } else {
-            if (split && l.half == 0 && !ntt0_marked) {
-                // half 0's NTT passes are all enqueued: half 1 may start
-                ZKX_HIP(hipEventRecord(ev_ntt0_, stream_));
-                ZKX_HIP(hipStreamWaitEvent(stream2_, ev_ntt0_, 0));
-                ntt0_marked = true;
-            }
-            launch_merkle_pass(s, l.hash, l.leaves, reinterpret_cast<const MerkleArgs *>(base), l.count,
-                               MerkleArgs{}, l.blocks, l.tile_log);
+            launch_merkle_pass(stream_, l.hash, reinterpret_cast<const MerkleArgs *>(base), l.count, MerkleArgs{},
+                               l.blocks, l.tile_log);
         }
-    }
-    if (split) {
-        ZKX_HIP(hipEventRecord(ev_done1_, stream2_));
-        ZKX_HIP(hipStreamWaitEvent(stream_, ev_done1_, 0));
     }
 }
 

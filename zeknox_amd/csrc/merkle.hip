@@ -81,22 +81,22 @@ uint64_t fuse_merkle_passes(const std::vector<MerkleArgs> &passes, MerkleArgs &o
     return ctrs;
 }
 
-void launch_merkle_pass(hipStream_t stream, int hash_type, bool from_leaves, const MerkleArgs *dev_list,
+void launch_merkle_pass(hipStream_t stream, int hash_type, const MerkleArgs *dev_list,
                         uint32_t n, const MerkleArgs &one, uint64_t blocks, uint32_t tile_log_max)
 {
     if (blocks == 0) return;
     if (blocks > 0x7FFFFFFFull) invalid("tree too large for one launch");
     if (tile_log_max < (uint32_t)kMerkleMinTileLog || tile_log_max > (uint32_t)kMerkleMaxTileLog)
         invalid("Merkle tile size out of range");
-    ProfScope ps(from_leaves ? "merkle_leaves" : "merkle_levels", stream);
+    ProfScope ps("merkle_leaves", stream);
     switch (hash_type) {
-    case kPoseidon2: launch_merkle_hash<kPoseidon2>(stream, from_leaves, dev_list, n, one, blocks, tile_log_max); break;
-    case kKeccak: launch_merkle_hash<kKeccak>(stream, from_leaves, dev_list, n, one, blocks, tile_log_max); break;
-    case kMonolith: launch_merkle_hash<kMonolith>(stream, from_leaves, dev_list, n, one, blocks, tile_log_max); break;
+    case kPoseidon2: launch_merkle_hash<kPoseidon2>(stream, dev_list, n, one, blocks, tile_log_max); break;
+    case kKeccak: launch_merkle_hash<kKeccak>(stream, dev_list, n, one, blocks, tile_log_max); break;
+    case kMonolith: launch_merkle_hash<kMonolith>(stream, dev_list, n, one, blocks, tile_log_max); break;
     case kPoseidonBN128:
-        launch_merkle_hash<kPoseidonBN128>(stream, from_leaves, dev_list, n, one, blocks, tile_log_max);
+        launch_merkle_hash<kPoseidonBN128>(stream, dev_list, n, one, blocks, tile_log_max);
         break;
-    default: launch_merkle_hash<kPoseidon>(stream, from_leaves, dev_list, n, one, blocks, tile_log_max); break;
+    default: launch_merkle_hash<kPoseidon>(stream, dev_list, n, one, blocks, tile_log_max); break;
     }
     ZKX_HIP(hipGetLastError());
 }
@@ -111,14 +111,14 @@ void merkle_build(Device &d, const MerkleSpec &s)
     const uint64_t nctr = fuse_merkle_passes(passes, f);
     f.tile_begin = 0;
     if (!nctr) {
-        launch_merkle_pass(d.stream, (int)s.hash_type, true, nullptr, 0, f, f.count >> f.tile_log, tl);
+        launch_merkle_pass(d.stream, (int)s.hash_type, nullptr, 0, f, f.count >> f.tile_log, tl);
         return;
     }
     // fused upper passes read their fields from a descriptor in device memory
     f.ctr = d.merkle_counters(nctr);
     MerkleArgs *dj = d.merkle_descriptor();
     ZKX_HIP(hipMemcpyAsync(dj, &f, sizeof(f), hipMemcpyHostToDevice, d.stream));
-    launch_merkle_pass(d.stream, (int)s.hash_type, true, dj, 1, MerkleArgs{}, f.count >> f.tile_log, tl);
+    launch_merkle_pass(d.stream, (int)s.hash_type, dj, 1, MerkleArgs{}, f.count >> f.tile_log, tl);
 }
 
 }  // namespace zkx

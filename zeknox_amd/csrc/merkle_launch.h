@@ -11,7 +11,7 @@ namespace zkx {
 // One launch of `blocks` work-groups of 2^tile_log_max lanes over the n
 // descriptors at dev_list (n > 0) or the single descriptor `one` (n == 0).
 template <int HT>
-void launch_merkle_hash(hipStream_t stream, bool from_leaves, const MerkleArgs *dev_list, uint32_t n,
-                        const MerkleArgs &one, uint64_t blocks, uint32_t tile_log_max);
+void launch_merkle_hash(hipStream_t stream, const MerkleArgs *dev_list, uint32_t n, const MerkleArgs &one,
+                        uint64_t blocks, uint32_t tile_log_max);
 
 }  // namespace zkx
