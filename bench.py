@@ -13,10 +13,12 @@ Two paths are timed:
   batched -- the batched commit engine (zeknox_amd.batch), commits of the
              same shape in flight together.
 The headline value is the batched rate; both, the per-kernel HIP-event
-timings, the roofline of the dominant kernel and a CPU baseline (the C
-oracle on host cores, bounded sample) go into one JSON line.
+timings, the roofline of the dominant kernel, a 64-job mixed batch and a
+large Poseidon2 batch both ways, a served stream, and a CPU baseline (the
+same commit on the host cores: the reference's own CPU Merkle and the
+oracle's LDE) go into one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--serve-seconds S]
 """
 import argparse
 import json
@@ -37,15 +39,16 @@ HASH = 0  # Poseidon
 HBM_PEAK_GBS = 8000.0
 
 # Per-unit figures of the leaf-hash launch on this workload, measured with
-# rocprofv3 --pmc on the batched 135 x 2^15 rate-3 launch (4 commits, 256-lane
-# work-groups) and kept in profiles/merkle_leaves_valu_135x2e18.txt and
+# rocprofv3 --pmc on the batched 135 x 2^15 rate-3 launch (4 commits) and
+# kept in profiles/merkle_leaves_valu_135x2e18.txt and
 # profiles/hbm_traffic_135x2e15_rate3.txt:
 #   VALU wave-instructions per Poseidon permutation (SQ_INSTS_VALU / perms,
-#   leaf loads and in-tile levels included)
-VALU_WAVE_INSTS_PER_PERM = 372.3
+#   leaf loads and in-tile levels included), by Merkle work-group size (log2
+#   lanes; larger tiles waste fewer lanes on the upper levels)
+VALU_WAVE_INSTS_PER_PERM = {8: 372.3, 9: 360.3, 10: 354.2}
 #   HBM bytes per commit: FETCH_SIZE x 2 (the counter reads half the bytes of
-#   a plain 1 GiB copy) + WRITE_SIZE x 1, KiB x 1024
-LEAF_TRAFFIC_BYTES_PER_COMMIT = 7.75e8
+#   a plain 1 GiB copy) + WRITE_SIZE x 1, KiB x 1024, by work-group size
+LEAF_TRAFFIC_BYTES_PER_COMMIT = {8: 7.75e8, 9: 8.35e8}
 #   NTT passes (both), same scaling
 NTT_TRAFFIC_BYTES_PER_COMMIT = 8.87e8
 # 32-bit VALU issue peak: 256 CUs x 4 SIMDs x one wave64 instruction per 4
@@ -384,15 +387,17 @@ def main():
     leaf = batch_kernels.get("merkle_leaves")
     roofline = None
     if leaf:
+        tl = schedule["merkle_tile_log"]
         perms = args.jobs * perms_per_commit(N, COLS, nc)
-        achieved = perms * VALU_WAVE_INSTS_PER_PERM * 64 / (leaf["avg_launch_ms"] * 1e-3) / 1e12
+        achieved = perms * VALU_WAVE_INSTS_PER_PERM[tl] * 64 / (leaf["avg_launch_ms"] * 1e-3) / 1e12
         alg_bytes = args.jobs * (8 * COLS * N * 2 + 32 * nd)  # LDE read, leaf rows + digests written
+        traffic = LEAF_TRAFFIC_BYTES_PER_COMMIT.get(tl)
         roofline = {"bound": "valu", "achieved": round(achieved, 2), "peak": round(VALU_PEAK_TOPS, 2),
                     "unit": "TOP/s (32-bit VALU lane-ops)", "frac": round(achieved / VALU_PEAK_TOPS, 3),
-                    "traffic": round(LEAF_TRAFFIC_BYTES_PER_COMMIT * args.jobs), "algorithmic_bytes": alg_bytes,
-                    "hbm_gbs": round(LEAF_TRAFFIC_BYTES_PER_COMMIT * args.jobs / (leaf["avg_launch_ms"] * 1e-3) / 1e9, 1),
+                    "traffic": round(traffic * args.jobs) if traffic else None, "algorithmic_bytes": alg_bytes,
+                    "hbm_gbs": round(traffic * args.jobs / (leaf["avg_launch_ms"] * 1e-3) / 1e9, 1) if traffic else None,
                     "kernel": "merkle_leaves", "avg_launch_ms": round(leaf["avg_launch_ms"], 4),
-                    "perms_per_launch": perms, "valu_wave_insts_per_perm": VALU_WAVE_INSTS_PER_PERM}
+                    "perms_per_launch": perms, "valu_wave_insts_per_perm": VALU_WAVE_INSTS_PER_PERM[tl]}
 
     # LDE passes against HBM: algorithmic bytes of the LDE (read n, write N
     # per column) over the batch's NTT launches
