@@ -66,6 +66,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="served-stream duration (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=40.0, help="served-stream arrivals per second")
+    ap.add_argument("--serve-trace", default=None,
+                    help="write one CSV line per served request (times in ms from the first arrival)")
     ap.add_argument("--no-tune", action="store_true", help="batched path on the default schedule (no autotuning)")
     ap.add_argument("--schedule", default=None, metavar="S,E,T",
                     help="batched 135 x 2^15 schedule (ntt_strided_max,ntt_even,merkle_tile_log) instead of autotuning")
@@ -227,7 +229,7 @@ def perms_per_commit(rows, cols, caps):
     return rows * per_leaf + (rows - caps)
 
 
-def served_stream(zk, np, seconds, rate_hz, budget, streams, seed=5):
+def served_stream(zk, np, seconds, rate_hz, budget, streams, seed=5, trace_path=None):
     """Poisson arrivals of heterogeneous commit requests into the commit server
     for `seconds`: log2 degree lognormal clipped to [12, 20], columns uniform
     in [2, 400], rate_bits in {1, 2, 3}, Poseidon / Poseidon2 50/50, cap
@@ -249,20 +251,28 @@ def served_stream(zk, np, seconds, rate_hz, budget, streams, seed=5):
         rate = int(rng.integers(1, 4))
         ht = 0 if rng.random() < 0.5 else 3
         arrivals.append((t, cols, lg_n, rate, ht))
-    ids = []
+    ids, shape_of = [], {}
     done = threading.Event()
-    lat, t_first, t_last = [], [None], [None]
+    lat, qwait, large, t_last, rows = [], [], [], [None], []
+    large_work = 1 << 28  # the server's small/large boundary, LDE elements
 
     def consumer():
-        i = 0
-        while i < len(arrivals):
-            while i >= len(ids):
+        # results are taken in completion order, as a client would
+        got = 0
+        while got < len(arrivals):
+            while got >= len(ids):
                 time.sleep(0.0005)
-            r = srv.wait(ids[i])
+            rid, r = srv.wait_any()
             lat.append(r.t_done - r.t_submit)
+            qwait.append(r.t_start - r.t_submit)
+            while rid not in shape_of:  # the submitting thread records it after submit returns
+                time.sleep(0.0001)
+            cols, lg_n, rate = shape_of[rid]
+            large.append((cols << (lg_n + rate)) > large_work)
+            rows.append((rid, r.t_submit, r.t_start, r.t_enqueued, r.t_done, cols, lg_n, rate))
             t_last[0] = r.t_done
-            srv.release(ids[i])
-            i += 1
+            srv.release(rid)
+            got += 1
         done.set()
 
     th = threading.Thread(target=consumer)
@@ -272,18 +282,33 @@ def served_stream(zk, np, seconds, rate_hz, budget, streams, seed=5):
         delay = t0 + ta - time.perf_counter()
         if delay > 0:
             time.sleep(delay)
-        ids.append(srv.submit(pool, cols, lg_n, rate, 4, ht))
+        rid = srv.submit(pool, cols, lg_n, rate, 4, ht)
+        shape_of[rid] = (cols, lg_n, rate)
+        ids.append(rid)
     done.wait()
     th.join()
     elapsed = time.perf_counter() - t0
     st = srv.stats()
     srv.close()
-    lat_ms = np.array(lat) * 1e3
+    lat_ms, q_ms, large = np.array(lat) * 1e3, np.array(qwait) * 1e3, np.array(large)
+    if trace_path:
+        t_first = min(row[1] for row in rows)
+        with open(trace_path, "w") as f:
+            f.write("id,submit_ms,start_ms,enqueued_ms,done_ms,cols,lg_n,rate\n")
+            for rid, ts, tst, te, td, cols, lg_n, rate in sorted(rows):
+                f.write(f"{rid},{(ts - t_first) * 1e3:.3f},{(tst - t_first) * 1e3:.3f},{(te - t_first) * 1e3:.3f},"
+                        f"{(td - t_first) * 1e3:.3f},{cols},{lg_n},{rate}\n")
+
+    def pct(a, q):
+        return round(float(np.percentile(a, q)), 2) if a.size else None
     return {"workload": f"Poisson {rate_hz}/s for {seconds} s; log2 degree lognormal(ln 14, 0.15) clipped [12,20], "
                         "cols U[2,400], rate_bits {1,2,3}, Poseidon/Poseidon2 50/50, cap_height 4",
             "requests": len(arrivals), "commits_per_s": round(len(arrivals) / elapsed, 2),
-            "p50_ms": round(float(np.percentile(lat_ms, 50)), 2), "p99_ms": round(float(np.percentile(lat_ms, 99)), 2),
-            "mean_ms": round(float(lat_ms.mean()), 2), "batches": st["batches"], "max_batch": st["max_batch"],
+            "p50_ms": pct(lat_ms, 50), "p99_ms": pct(lat_ms, 99), "mean_ms": round(float(lat_ms.mean()), 2),
+            "queue_p50_ms": pct(q_ms, 50), "queue_p99_ms": pct(q_ms, 99),
+            "small_p50_ms": pct(lat_ms[~large], 50), "small_p99_ms": pct(lat_ms[~large], 99),
+            "large_requests": int(large.sum()), "large_p50_ms": pct(lat_ms[large], 50),
+            "large_p99_ms": pct(lat_ms[large], 99), "batches": st["batches"], "max_batch": st["max_batch"],
             "peak_hbm_gb": round(st["peak_bytes"] / 2**30, 2), "budget_gb": round(budget / 2**30, 1),
             "streams": streams}
 
@@ -425,7 +450,8 @@ def main():
 
     served = None
     if args.serve_seconds > 0:
-        served = served_stream(zk, np, args.serve_seconds, args.serve_rate, 64 << 30, 3)
+        served = served_stream(zk, np, args.serve_seconds, args.serve_rate, 64 << 30, 3,
+                               trace_path=args.serve_trace)
 
     value = args.jobs / batch_step_s
     out = {

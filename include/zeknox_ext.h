@@ -97,15 +97,22 @@ RustError zkx_tuning_load(size_t device_id, const char *path);
 
 /* ---- commit server (continuous batching under an HBM budget) ------------
  * Requests are commit jobs as above whose output pointers are ignored: the
- * server allocates leaf rows (if want_leaves), digests and caps on the device
- * and keeps them until zkx_server_release.  Requests that arrive while others
- * run are admitted as soon as an engine (stream) is free and their working
- * set fits the budget next to everything admitted and not yet released. */
+ * server places leaf rows (if want_leaves), digests and caps in device memory
+ * it allocated once at creation (budget_bytes) and keeps them until
+ * zkx_server_release; serving itself allocates nothing.  Requests that
+ * arrive while others run are admitted as soon as an engine (stream) is free
+ * and their working set can be placed next to everything admitted and not
+ * yet released.
+ * Small requests (LDE of at most 2^28 elements) are packed into batches of
+ * at most 2^28 elements; large ones run one per batch on at most nstreams - 1
+ * engines, and the oldest large request keeps its room in the budget
+ * against later small ones. */
 typedef struct {
     void *leaves, *digests, *caps;  /* device outputs (leaves NULL if not wanted) */
-    double t_submit, t_start, t_done; /* steady-clock seconds */
+    double t_submit, t_start, t_done; /* steady-clock seconds: submitted, admitted, finished */
     int state;                        /* 2 done, 3 failed */
     int reserved;
+    double t_enqueued;                /* the request's kernels were enqueued */
 } ZkxServeResult;
 
 typedef struct {
@@ -120,6 +127,9 @@ RustError zkx_server_destroy(void *server);
 size_t zkx_server_footprint(const ZkxCommitJob *job, int want_leaves);
 RustError zkx_server_submit(void *server, const ZkxCommitJob *job, int want_leaves, uint64_t *id);
 RustError zkx_server_wait(void *server, uint64_t id, ZkxServeResult *result);
+/* Blocks until a request finishes that no zkx_server_wait / _wait_any call
+ * has returned yet; writes its id.  Code 1 when no request is pending. */
+RustError zkx_server_wait_any(void *server, uint64_t *id, ZkxServeResult *result);
 RustError zkx_server_release(void *server, uint64_t id);
 RustError zkx_server_stats(void *server, ZkxServeStats *stats);
 

@@ -102,3 +102,60 @@ def test_request_over_budget_is_rejected(zk):
     with pytest.raises(zk.ZeknoxError):
         srv.submit(d, 135, 12, 3, 4, 0)
     srv.close()
+
+
+def test_wait_any_returns_each_request_once(zk):
+    srv = zk.Server(2 << 30, streams=2, max_batch=3)
+    shapes = SHAPES[:5]
+    inputs = [ob.canonical(c << l, 300 + i) for i, (c, l, r, h, ht) in enumerate(shapes)]
+    dev = [zk.DeviceBuffer.from_host(x) for x in inputs]
+    ids = {srv.submit(d, c, l, r, h, ht): k for k, (d, (c, l, r, h, ht)) in enumerate(zip(dev, shapes))}
+    seen = set()
+    for _ in shapes:
+        rid, res = srv.wait_any()
+        assert rid in ids and rid not in seen and res.state == 2
+        seen.add(rid)
+        k = ids[rid]
+        c, l, r, h, ht = shapes[k]
+        for a, b in zip(srv.results(res, c, l, r, h), alone(zk, inputs[k], c, l, r, h, ht)):
+            assert np.array_equal(a, b)
+        srv.release(rid)
+    with pytest.raises(zk.ZeknoxError):
+        srv.wait_any()  # nothing pending
+    srv.close()
+
+
+def test_small_requests_run_beside_a_large_one(zk):
+    """A large request (LDE above 2^28 elements) runs alone on one engine;
+    small requests submitted after it start on the other engine before it
+    ends, and every result equals the same job run alone."""
+    big = (135, 19, 2, 4, 0)  # 135 x 2^21 LDE elements
+    c, l, r, h, ht = big
+    assert (c << (l + r)) > (1 << 28)
+    xb = zk.DeviceBuffer(c << l)
+    xb.fill_random(31)
+    small = [(20, 12, 2, 4, 3), (9, 10, 1, 3, 0), (33, 11, 3, 4, 0)]
+    xs = [ob.canonical(sc << sl, 60 + i) for i, (sc, sl, _, _, _) in enumerate(small)]
+    ds = [zk.DeviceBuffer.from_host(x) for x in xs]
+    srv = zk.Server(24 << 30, streams=2, max_batch=8)
+    bid = srv.submit(xb, c, l, r, h, ht)
+    sids = [srv.submit(d, *s) for d, s in zip(ds, small)]
+    starts = []
+    for rid, x, s in zip(sids, xs, small):
+        res = srv.wait(rid)
+        starts.append(res.t_start)
+        got = srv.results(res, *s[:4])
+        for a, b in zip(got, alone(zk, x, *s)):
+            assert np.array_equal(a, b)
+        srv.release(rid)
+    bres = srv.wait(bid)
+    assert max(starts) < bres.t_done
+    got = srv.results(bres, c, l, r, h)
+    srv.release(bid)
+    ref = zk.Commit(xb, c, l, r, h, ht)
+    zk.commit_batch([ref])
+    for a, b in zip(got, ref.results()):
+        assert np.array_equal(a, b)
+    st = srv.stats()
+    assert st["completed"] == 4 and st["failed"] == 0 and st["bytes_in_use"] == 0
+    srv.close()

@@ -103,7 +103,7 @@ class ServeResult(ctypes.Structure):
     """zeknox_ext.h ZkxServeResult."""
     _fields_ = [("leaves", ctypes.c_void_p), ("digests", ctypes.c_void_p), ("caps", ctypes.c_void_p),
                 ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_done", ctypes.c_double),
-                ("state", ctypes.c_int), ("reserved", ctypes.c_int)]
+                ("state", ctypes.c_int), ("reserved", ctypes.c_int), ("t_enqueued", ctypes.c_double)]
 
 
 class ServeStats(ctypes.Structure):
@@ -162,6 +162,7 @@ def _load():
         "zkx_server_footprint": (sz, [ctypes.POINTER(CommitJob), ctypes.c_int]),
         "zkx_server_submit": (RustError, [vp, ctypes.POINTER(CommitJob), ctypes.c_int, ctypes.POINTER(u64)]),
         "zkx_server_wait": (RustError, [vp, u64, ctypes.POINTER(ServeResult)]),
+        "zkx_server_wait_any": (RustError, [vp, ctypes.POINTER(u64), ctypes.POINTER(ServeResult)]),
         "zkx_server_release": (RustError, [vp, u64]),
         "zkx_server_stats": (RustError, [vp, ctypes.POINTER(ServeStats)]),
     }
@@ -436,6 +437,12 @@ class Server:
         r = ServeResult()
         _check(lib.zkx_server_wait(self.h, rid, ctypes.byref(r)))
         return r
+
+    def wait_any(self):
+        """Next finished request not yet returned by wait/wait_any: (id, result)."""
+        rid, r = ctypes.c_uint64(), ServeResult()
+        _check(lib.zkx_server_wait_any(self.h, ctypes.byref(rid), ctypes.byref(r)))
+        return rid.value, r
 
     def results(self, r, cols, lg_n, rate, cap_height):
         """Copy a finished request's outputs to the host: (leaves, digests, caps)."""

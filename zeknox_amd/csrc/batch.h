@@ -52,7 +52,10 @@ public:
     // stream-ordered on this engine's stream) replaces the engine's own
     // working buffer when given.
     // `forced`, when given, replaces every job's schedule parameters.
-    void commit(const CommitJob *jobs, size_t n, void *scratch = nullptr, const Tuning *forced = nullptr);
+    // `lde_bufs`, when given, holds one working buffer per job instead
+    // (job i: cols << (lg_n + rate) elements, 256-byte aligned).
+    void commit(const CommitJob *jobs, size_t n, void *scratch = nullptr, const Tuning *forced = nullptr,
+                uint64_t *const *lde_bufs = nullptr);
     std::mutex mu;
     void sync();
     // Wait for the engine's work, then free its working buffer (it grows

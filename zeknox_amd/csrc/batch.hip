@@ -68,7 +68,7 @@ void Engine::trim()
     scratch_ = DevBuf();
 }
 
-void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning *forced)
+void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning *forced, uint64_t *const *lde_bufs)
 {
     if (n == 0) return;
     for (size_t i = 0; i < n; i++) check_commit_job(jobs[i]);
@@ -89,14 +89,20 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
     }
 
     // ---- working memory: every job's LDE in DIF (bit-reversed) order
-    const size_t need = commit_scratch_bytes(jobs, n);
-    if (!scratch && scratch_.bytes < need) {
-        sync();
-        scratch_ = DevBuf(need);
-    }
-    char *base = static_cast<char *>(scratch ? scratch : scratch_.p);
     std::vector<uint64_t *> lde(n);
-    {
+    if (lde_bufs) {
+        for (size_t i = 0; i < n; i++) {
+            if (!lde_bufs[i] || (reinterpret_cast<uintptr_t>(lde_bufs[i]) & (kAlign - 1)))
+                invalid("commit: working buffers must be 256-byte aligned");
+            lde[i] = lde_bufs[i];
+        }
+    } else {
+        const size_t need = commit_scratch_bytes(jobs, n);
+        if (!scratch && scratch_.bytes < need) {
+            sync();
+            scratch_ = DevBuf(need);
+        }
+        char *base = static_cast<char *>(scratch ? scratch : scratch_.p);
         size_t off = 0;
         for (size_t i = 0; i < n; i++) {
             lde[i] = reinterpret_cast<uint64_t *>(base + off);

@@ -521,22 +521,37 @@ RustError zkx_server_submit(void *server, const ZkxCommitJob *job, int want_leav
     });
 }
 
+static void fill_result(const ServeRequest &r, ZkxServeResult *result)
+{
+    if (result) {
+        result->leaves = r.job.leaves;
+        result->digests = r.job.digests;
+        result->caps = r.job.caps;
+        result->t_submit = r.t_submit;
+        result->t_start = r.t_start;
+        result->t_done = r.t_done;
+        result->state = r.state;
+        result->reserved = 0;
+        result->t_enqueued = r.t_enqueued;
+    }
+    if (r.state == 3) throw Error(-1, "request failed: " + r.error);
+}
+
 RustError zkx_server_wait(void *server, uint64_t id, ZkxServeResult *result)
 {
     return guarded([&] {
         if (!server) invalid("null server");
-        ServeRequest r = static_cast<Server *>(server)->wait(id);
-        if (result) {
-            result->leaves = r.job.leaves;
-            result->digests = r.job.digests;
-            result->caps = r.job.caps;
-            result->t_submit = r.t_submit;
-            result->t_start = r.t_start;
-            result->t_done = r.t_done;
-            result->state = r.state;
-            result->reserved = 0;
-        }
-        if (r.state == 3) throw Error(-1, "request failed: " + r.error);
+        fill_result(static_cast<Server *>(server)->wait(id), result);
+    });
+}
+
+RustError zkx_server_wait_any(void *server, uint64_t *id, ZkxServeResult *result)
+{
+    return guarded([&] {
+        if (!server || !id) invalid("null argument");
+        ServeRequest r = static_cast<Server *>(server)->wait_any();
+        *id = r.id;
+        fill_result(r, result);
     });
 }
 

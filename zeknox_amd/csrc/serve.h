@@ -1,16 +1,34 @@
 // Commit server: continuous batching of LDE + Merkle commit requests on one
 // device under an HBM budget.
 //
-// Requests arrive at any time (submit() from any thread).  A scheduler thread
-// owns `nstreams` commit engines (one stream each).  Whenever an engine is
-// free it admits the oldest queued requests whose working set -- LDE scratch
-// plus output buffers (leaf rows, digests, caps) -- still fits in the budget
-// next to everything admitted and not yet released, and launches them as one
-// batch; batches on different engines overlap on the device.  Outputs live in
-// device memory from a stream-ordered pool until the client releases the
-// request, so a request's result is the same alone or in any mix (each job's
-// kernels touch only its own buffers).  Twiddle, coset and LDE factor tables
-// stay resident on the device between requests.
+// Requests arrive at any time (submit() from any thread).  The server owns
+// `nstreams` commit engines (one stream and one completion thread each).
+// Admission runs whenever a request arrives, a batch completes or a client
+// releases a result, and hands batches to free engines:
+//
+//   * small requests (LDE of at most kLargeWork elements) are taken in
+//     arrival order and packed into batches of at most kLargeWork elements
+//     in total and max_batch_jobs jobs, so a small request never waits for a
+//     large one in its own batch;
+//   * large requests run one per batch, in arrival order among themselves,
+//     on at most nstreams - 1 engines at once: one engine always stays free
+//     for small batches;
+//   * the budget is one device allocation made when the server starts; a
+//     request is admitted only when its working set -- output buffers (leaf
+//     rows, digests, caps) followed by its LDE scratch -- can be placed in it
+//     next to everything admitted and not yet released.  Large requests are
+//     placed from the top of the region, small ones from the bottom.  While
+//     the oldest large request waits for room, small requests are placed
+//     only below the top `bytes` of the region that request needs, so it
+//     starts as soon as the large requests ahead of it finish: no request
+//     starves.
+//
+// Serving never allocates device memory: placement is host bookkeeping (a
+// request's scratch returns to the region when its batch completes, its
+// outputs when the client releases it).  Batches on different engines
+// overlap on the device.  A request's result is the same alone or in any mix
+// (each job's kernels touch only its own buffers).  Twiddle, coset and LDE
+// factor tables stay resident on the device between requests.
 #pragma once
 #include <condition_variable>
 #include <deque>
@@ -21,6 +39,7 @@
 #include <vector>
 
 #include "batch.h"
+#include "placement.h"
 
 namespace zkx {
 
@@ -31,9 +50,13 @@ struct ServeRequest {
     CommitJob job;
     bool want_leaves = true;
     // filled by the server
-    size_t bytes = 0;  // budget charge while admitted
-    void *out = nullptr;
-    double t_submit = 0, t_start = 0, t_done = 0;  // seconds, steady clock
+    bool large = false;
+    bool collected = false;  // returned by wait() or wait_any()
+    size_t bytes = 0;        // working set: outputs + scratch
+    size_t out_bytes = 0;    // outputs, at the start of the working set
+    size_t off = 0;          // working set's offset in the region
+    // seconds, steady clock: submitted, admitted, kernels enqueued, finished
+    double t_submit = 0, t_start = 0, t_enqueued = 0, t_done = 0;
     int state = 0;                                  // 0 queued, 1 running, 2 done, 3 failed
     std::string error;
 };
@@ -46,6 +69,11 @@ struct ServeStats {
 
 class Server {
 public:
+    // LDE elements (cols x 2^(lg_n + rate)) above which a request is large;
+    // also the work cap of a batch of small requests (about 20 ms of device
+    // time at the batched rate).
+    static constexpr uint64_t kLargeWork = 1ull << 28;
+
     Server(Device &d, size_t budget_bytes, int nstreams, size_t max_batch_jobs);
     ~Server();
 
@@ -54,38 +82,50 @@ public:
     uint64_t submit(const CommitJob &job, bool want_leaves);
     // Blocks until the request is done (or failed); returns its record.
     ServeRequest wait(uint64_t id);
+    // Blocks until some finished request that no wait()/wait_any() call has
+    // returned yet exists; returns it.  Throws Error(1) when none is pending.
+    ServeRequest wait_any();
     bool poll(uint64_t id, ServeRequest *out);
     // Frees the request's output buffers and its budget charge.
     void release(uint64_t id);
     ServeStats stats();
     static size_t footprint(const CommitJob &job, bool want_leaves);
+    static uint64_t work(const CommitJob &job);
     static double now();
 
 private:
-    void loop();
-    void launch(size_t engine, std::vector<ServeRequest *> batch);
+    struct Slot {
+        std::vector<ServeRequest *> batch;
+        hipEvent_t done = nullptr;
+        bool busy = false;      // batch assigned and not yet completed
+        bool assigned = false;  // batch handed over, completion thread not yet started on it
+        bool large = false;
+        std::condition_variable cv;
+        std::thread th;
+    };
+
+    void admit_locked();
+    void assign_locked(size_t e, std::vector<ServeRequest *> batch, bool large);
+    void run(size_t e);
+    bool launch(size_t e, std::vector<ServeRequest *> &batch, std::string &err);
 
     Device &dev_;
     size_t budget_;
     size_t max_batch_;
     std::vector<std::unique_ptr<Engine>> engines_;
-    struct Slot {
-        std::vector<ServeRequest *> batch;
-        void *scratch = nullptr;
-        hipEvent_t done = nullptr;
-        bool busy = false;
-    };
-    std::vector<Slot> slots_;
+    std::vector<std::unique_ptr<Slot>> slots_;
+    DevBuf region_;     // the budget, allocated once
+    Placement place_;
     std::mutex mu_;
-    std::condition_variable cv_;       // scheduler wake-up
     std::condition_variable done_cv_;  // request completion
-    std::deque<ServeRequest *> queue_;
+    std::deque<ServeRequest *> small_q_, large_q_;
+    std::deque<uint64_t> finished_;    // completion order, for wait_any
     std::map<uint64_t, std::unique_ptr<ServeRequest>> reqs_;
     uint64_t next_id_ = 1;
     size_t used_ = 0;
+    int large_running_ = 0;
     ServeStats stats_;
     bool stop_ = false;
-    std::thread worker_;
 };
 
 }  // namespace zkx

@@ -40,23 +40,24 @@ size_t Server::footprint(const CommitJob &job, bool want_leaves)
     return commit_scratch_bytes(&job, 1) + out_layout(job, want_leaves).total;
 }
 
+uint64_t Server::work(const CommitJob &job) { return (uint64_t)job.cols << (job.lg_n + job.rate); }
+
+// ---- server ---------------------------------------------------------------------
+
 Server::Server(Device &d, size_t budget_bytes, int nstreams, size_t max_batch_jobs)
-    : dev_(d), budget_(budget_bytes), max_batch_(max_batch_jobs ? max_batch_jobs : 64)
+    : dev_(d), budget_(budget_bytes & ~(kAlign - 1)), max_batch_(max_batch_jobs ? max_batch_jobs : 64)
 {
     if (nstreams < 1 || nstreams > 8) invalid("server: 1..8 streams");
+    if (budget_ == 0) invalid("server: budget below 256 bytes");
     ZKX_HIP(hipSetDevice(d.id));
-    // keep freed pool memory cached on the device between batches
-    hipMemPool_t pool;
-    ZKX_HIP(hipDeviceGetDefaultMemPool(&pool, d.id));
-    uint64_t threshold = UINT64_MAX;
-    ZKX_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold));
+    region_ = DevBuf(budget_);
+    place_.reset(budget_);
     for (int i = 0; i < nstreams; i++) {
         engines_.push_back(std::make_unique<Engine>(d));
-        Slot s;
-        ZKX_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
-        slots_.push_back(s);
+        slots_.push_back(std::make_unique<Slot>());
+        ZKX_HIP(hipEventCreateWithFlags(&slots_.back()->done, hipEventDisableTiming));
     }
-    worker_ = std::thread([this] { loop(); });
+    for (size_t e = 0; e < slots_.size(); e++) slots_[e]->th = std::thread([this, e] { run(e); });
 }
 
 Server::~Server()
@@ -65,14 +66,13 @@ Server::~Server()
         std::lock_guard<std::mutex> g(mu_);
         stop_ = true;
     }
-    cv_.notify_all();
-    if (worker_.joinable()) worker_.join();
+    // each completion thread finishes the batch it holds, then exits
+    for (auto &s : slots_) s->cv.notify_all();
+    for (auto &s : slots_)
+        if (s->th.joinable()) s->th.join();
     (void)hipSetDevice(dev_.id);
     for (auto &e : engines_) (void)hipStreamSynchronize(e->stream());
-    for (auto &kv : reqs_)
-        if (kv.second->out) (void)hipFreeAsync(kv.second->out, engines_[0]->stream());
-    for (auto &e : engines_) (void)hipStreamSynchronize(e->stream());
-    for (auto &s : slots_) (void)hipEventDestroy(s.done);
+    for (auto &s : slots_) (void)hipEventDestroy(s->done);
 }
 
 uint64_t Server::submit(const CommitJob &job, bool want_leaves)
@@ -85,16 +85,15 @@ uint64_t Server::submit(const CommitJob &job, bool want_leaves)
     r->job = job;
     r->want_leaves = want_leaves;
     r->bytes = bytes;
+    r->out_bytes = out_layout(job, want_leaves).total;
+    r->large = work(job) > kLargeWork;
     r->t_submit = now();
-    uint64_t id;
-    {
-        std::lock_guard<std::mutex> g(mu_);
-        id = r->id = next_id_++;
-        queue_.push_back(r.get());
-        reqs_[id] = std::move(r);
-        stats_.submitted++;
-    }
-    cv_.notify_all();
+    std::lock_guard<std::mutex> g(mu_);
+    const uint64_t id = r->id = next_id_++;
+    (r->large ? large_q_ : small_q_).push_back(r.get());
+    reqs_[id] = std::move(r);
+    stats_.submitted++;
+    admit_locked();
     return id;
 }
 
@@ -115,7 +114,25 @@ ServeRequest Server::wait(uint64_t id)
     if (it == reqs_.end()) invalid("server: unknown request");
     ServeRequest *r = it->second.get();
     done_cv_.wait(g, [&] { return r->state >= 2; });
+    r->collected = true;
     return *r;
+}
+
+ServeRequest Server::wait_any()
+{
+    std::unique_lock<std::mutex> g(mu_);
+    while (true) {
+        while (!finished_.empty()) {
+            const uint64_t id = finished_.front();
+            finished_.pop_front();
+            auto it = reqs_.find(id);
+            if (it == reqs_.end() || it->second->collected) continue;
+            it->second->collected = true;
+            return *it->second;
+        }
+        if (stats_.submitted == stats_.completed + stats_.failed) invalid("server: no request pending");
+        done_cv_.wait(g);
+    }
 }
 
 void Server::release(uint64_t id)
@@ -128,14 +145,13 @@ void Server::release(uint64_t id)
         if (it->second->state < 2) invalid("server: request still running");
         r = std::move(it->second);
         reqs_.erase(it);
-        used_ -= out_layout(r->job, r->want_leaves).total;
+        // the request's scratch went back when its batch completed (or
+        // failed); its outputs go back now
+        place_.give(r->off, r->out_bytes);
+        used_ -= r->out_bytes;
         stats_.bytes_in_use = used_;
+        admit_locked();  // room freed: queued requests may fit now
     }
-    if (r->out) {
-        ZKX_HIP(hipSetDevice(dev_.id));
-        ZKX_HIP(hipFreeAsync(r->out, engines_[0]->stream()));
-    }
-    cv_.notify_all();  // budget freed: queued requests may fit now
 }
 
 ServeStats Server::stats()
@@ -144,106 +160,149 @@ ServeStats Server::stats()
     return stats_;
 }
 
-void Server::launch(size_t e, std::vector<ServeRequest *> batch)
+void Server::assign_locked(size_t e, std::vector<ServeRequest *> batch, bool large)
+{
+    Slot &s = *slots_[e];
+    const double t = now();
+    for (ServeRequest *r : batch) {
+        r->state = 1;
+        r->t_start = t;
+    }
+    stats_.batches++;
+    stats_.max_batch = std::max<uint64_t>(stats_.max_batch, batch.size());
+    stats_.bytes_in_use = used_;
+    stats_.peak_bytes = std::max(stats_.peak_bytes, used_);
+    s.batch = std::move(batch);
+    s.large = large;
+    s.busy = true;
+    s.assigned = true;
+    s.cv.notify_one();
+}
+
+void Server::admit_locked()
+{
+    if (stop_) return;
+    auto free_slot = [&]() -> long {
+        for (size_t e = 0; e < slots_.size(); e++)
+            if (!slots_[e]->busy) return (long)e;
+        return -1;
+    };
+    // large requests: arrival order, one per batch, placed from the top of
+    // the region, leaving one engine to small batches (when there is more
+    // than one)
+    const int lanes = slots_.size() > 1 ? (int)slots_.size() - 1 : 1;
+    while (!large_q_.empty() && large_running_ < lanes) {
+        ServeRequest *r = large_q_.front();
+        const long e = free_slot();
+        if (e < 0 || !place_.take_high(r->bytes, r->off)) break;
+        large_q_.pop_front();
+        used_ += r->bytes;
+        large_running_++;
+        assign_locked((size_t)e, {r}, true);
+    }
+    // small requests: arrival order, packed up to kLargeWork elements, placed
+    // from the bottom; while a large request waits, only below the room it
+    // needs at the top
+    const ServeRequest *blocked = large_q_.empty() ? nullptr : large_q_.front();
+    const size_t limit = blocked ? budget_ - blocked->bytes : budget_;
+    while (!small_q_.empty()) {
+        const long e = free_slot();
+        if (e < 0) break;
+        std::vector<ServeRequest *> batch;
+        uint64_t w = 0;
+        while (!small_q_.empty() && batch.size() < max_batch_) {
+            ServeRequest *r = small_q_.front();
+            const uint64_t rw = work(r->job);
+            if (!batch.empty() && w + rw > kLargeWork) break;
+            if (!place_.take_low(r->bytes, limit, r->off)) break;
+            small_q_.pop_front();
+            used_ += r->bytes;
+            w += rw;
+            batch.push_back(r);
+        }
+        if (batch.empty()) break;
+        assign_locked((size_t)e, std::move(batch), false);
+    }
+}
+
+bool Server::launch(size_t e, std::vector<ServeRequest *> &batch, std::string &err)
 {
     Engine &eng = *engines_[e];
-    Slot &slot = slots_[e];
+    Slot &slot = *slots_[e];
     hipStream_t st = eng.stream();
-    std::vector<CommitJob> jobs;
     try {
         ZKX_HIP(hipSetDevice(dev_.id));
+        std::vector<CommitJob> jobs;
+        std::vector<uint64_t *> lde;
         for (ServeRequest *r : batch) {
             const OutLayout o = out_layout(r->job, r->want_leaves);
-            ZKX_HIP(hipMallocAsync(&r->out, o.total, st));
-            char *p = static_cast<char *>(r->out);
+            char *p = region_.as<char>() + r->off;
             r->job.leaves = r->want_leaves ? reinterpret_cast<uint64_t *>(p) : nullptr;
             r->job.digests = reinterpret_cast<uint64_t *>(p + o.leaves);
             r->job.caps = reinterpret_cast<uint64_t *>(p + o.leaves + o.digests);
             jobs.push_back(r->job);
+            lde.push_back(reinterpret_cast<uint64_t *>(p + r->out_bytes));
         }
-        const size_t sbytes = commit_scratch_bytes(jobs.data(), jobs.size());
-        ZKX_HIP(hipMallocAsync(&slot.scratch, sbytes, st));
         {
             std::lock_guard<std::mutex> g(eng.mu);
-            eng.commit(jobs.data(), jobs.size(), slot.scratch);
+            eng.commit(jobs.data(), jobs.size(), nullptr, nullptr, lde.data());
         }
-        ZKX_HIP(hipFreeAsync(slot.scratch, st));
-        slot.scratch = nullptr;
         ZKX_HIP(hipEventRecord(slot.done, st));
+        const double t = now();
+        for (ServeRequest *r : batch) r->t_enqueued = t;
+        return true;
     } catch (const std::exception &ex) {
         (void)hipStreamSynchronize(st);
-        if (slot.scratch) (void)hipFreeAsync(slot.scratch, st);
-        slot.scratch = nullptr;
-        std::lock_guard<std::mutex> g(mu_);
-        for (ServeRequest *r : batch) {
-            r->state = 3;
-            r->error = ex.what();
-            r->t_done = now();
-            used_ -= commit_scratch_bytes(&r->job, 1);
-            stats_.failed++;
-        }
-        slot.busy = false;
-        done_cv_.notify_all();
-        return;
+        err = ex.what();
+        return false;
     }
-    const double t = now();
-    std::lock_guard<std::mutex> g(mu_);
-    for (ServeRequest *r : batch) r->t_start = t;
-    slot.batch = std::move(batch);
 }
 
-void Server::loop()
+void Server::run(size_t e)
 {
+    Slot &s = *slots_[e];
     (void)hipSetDevice(dev_.id);
+    std::unique_lock<std::mutex> g(mu_);
     while (true) {
-        std::vector<std::pair<size_t, std::vector<ServeRequest *>>> to_launch;
-        {
-            std::unique_lock<std::mutex> g(mu_);
-            bool any_busy = false;
-            for (auto &s : slots_) any_busy |= s.busy;
-            const bool idle = queue_.empty() && !any_busy;
-            if (stop_ && idle) return;
-            // nothing running: sleep until a submit; else poll completions often
-            if (idle) cv_.wait(g, [&] { return stop_ || !queue_.empty(); });
-            else cv_.wait_for(g, std::chrono::microseconds(100));
-
-            // completions
-            for (auto &s : slots_) {
-                if (!s.busy || s.batch.empty()) continue;
-                if (hipEventQuery(s.done) != hipSuccess) continue;
-                const double t = now();
-                for (ServeRequest *r : s.batch) {
-                    r->state = 2;
-                    r->t_done = t;
-                    used_ -= commit_scratch_bytes(&r->job, 1);
-                    stats_.completed++;
-                }
-                s.batch.clear();
-                s.busy = false;
-                done_cv_.notify_all();
-            }
-            // admission: FIFO, as many queued requests as fit the budget
-            for (size_t e = 0; e < slots_.size(); e++) {
-                if (slots_[e].busy || queue_.empty()) continue;
-                std::vector<ServeRequest *> batch;
-                while (!queue_.empty() && batch.size() < max_batch_) {
-                    ServeRequest *r = queue_.front();
-                    if (used_ + r->bytes > budget_) break;
-                    queue_.pop_front();
-                    used_ += r->bytes;
-                    r->state = 1;
-                    batch.push_back(r);
-                }
-                if (batch.empty()) break;
-                slots_[e].busy = true;
-                stats_.batches++;
-                stats_.max_batch = std::max<uint64_t>(stats_.max_batch, batch.size());
-                stats_.bytes_in_use = used_;
-                stats_.peak_bytes = std::max(stats_.peak_bytes, used_);
-                to_launch.emplace_back(e, std::move(batch));
+        s.cv.wait(g, [&] { return stop_ || s.assigned; });
+        if (!s.assigned) return;
+        s.assigned = false;
+        std::vector<ServeRequest *> batch = s.batch;
+        g.unlock();
+        // enqueue, then wait for the batch on this thread: completion is seen
+        // as soon as the device signals it
+        std::string err;
+        bool ok = launch(e, batch, err);
+        if (ok) {
+            const hipError_t rc = hipEventSynchronize(s.done);
+            if (rc != hipSuccess) {
+                ok = false;
+                err = hipGetErrorString(rc);
             }
         }
-        for (auto &l : to_launch) launch(l.first, std::move(l.second));
+        g.lock();
+        const double t = now();
+        for (ServeRequest *r : batch) {
+            const size_t sb = r->bytes - r->out_bytes;
+            place_.give(r->off + r->out_bytes, sb);
+            used_ -= sb;
+            r->t_done = t;
+            if (ok) {
+                r->state = 2;
+                stats_.completed++;
+            } else {
+                r->state = 3;
+                r->error = err;
+                stats_.failed++;
+            }
+            finished_.push_back(r->id);
+        }
+        if (s.large) large_running_--;
+        s.batch.clear();
+        s.busy = false;
+        stats_.bytes_in_use = used_;
+        admit_locked();
+        done_cv_.notify_all();
     }
 }
 
