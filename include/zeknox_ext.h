@@ -31,6 +31,14 @@ RustError zkx_device_synchronize(size_t device_id);
  * until below p, so uniform canonical field elements (benchmarks). */
 RustError zkx_fill_random(size_t device_id, void *dst, size_t count, uint64_t seed);
 
+/* Goldilocks arithmetic on device vectors through the library's own device
+ * routines (those the hash and NTT kernels use): out[i] = op(a[i], b[i]),
+ * canonical.  op 0: a * b (lazy product, any a, b < 2^64); 1: a * b;
+ * 2: b * 2^64 + a (any 128-bit value); 3: (b mod 2^32) * 2^64 + a; 4: a^7
+ * (any a < 2^64); 5: a + b (a, b < p); 6: a - b (a, b < p); 7: a mod p.
+ * No counterpart in the reference (its field tests are C++ unit tests). */
+RustError zkx_gl_ops(size_t device_id, int op, const void *a, const void *b, void *out, size_t n);
+
 /* The stream the library enqueues its work on (a hipStream_t). */
 void *zkx_stream(size_t device_id);
 

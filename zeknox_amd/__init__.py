@@ -141,6 +141,7 @@ def _load():
         "zkx_memcpy_dtoh": (RustError, [sz, vp, vp, sz]),
         "zkx_memset": (RustError, [sz, vp, ctypes.c_int, sz]),
         "zkx_fill_random": (RustError, [sz, vp, sz, u64]),
+        "zkx_gl_ops": (RustError, [sz, ctypes.c_int, vp, vp, vp, sz]),
         "zkx_commit_trim": (RustError, [sz]),
         "zkx_autotune": (RustError, [sz, vp, sz, ctypes.c_int]),
         "zkx_get_tuning": (RustError, [sz, vp, ctypes.POINTER(Tuning)]),
@@ -355,6 +356,21 @@ class Commit:
         leaves = self.d_leaves.to_host() if self.d_leaves else None
         dig = self.d_digests.to_host(self.num_digests * 4) if self.num_digests else np.zeros(0, np.uint64)
         return leaves, dig, self.d_caps.to_host()
+
+
+GL_MUL_LAZY, GL_MUL, GL_REDUCE128, GL_REDUCE96, GL_POW7, GL_ADD, GL_SUB, GL_CANON = range(8)
+
+
+def gl_ops(op, a, b, device_id=0):
+    """Elementwise Goldilocks op on the device (zeknox_ext.h zkx_gl_ops):
+    numpy uint64 arrays in, canonical numpy uint64 array out."""
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    b = np.ascontiguousarray(b, dtype=np.uint64)
+    assert a.shape == b.shape
+    da, db = DeviceBuffer.from_host(a, device_id), DeviceBuffer.from_host(b, device_id)
+    out = DeviceBuffer(a.size, device_id)
+    _check(lib.zkx_gl_ops(device_id, op, da.ptr, db.ptr, out.ptr, a.size))
+    return out.to_host()
 
 
 def commit_batch(commits, device_id=0):

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/zeknox_ext.h"
+#include "gl64.h"
 #include "ntt.h"
 #include "runtime.h"
 
@@ -151,6 +152,47 @@ RustError zkx_fill_random(size_t device_id, void *dst, size_t count, uint64_t se
         if (count && !dst) invalid("null pointer");
         if (count > (0x7FFFFFFFull << 8)) invalid("fill too large for one launch");
         fill_salt(static_cast<uint64_t *>(dst), count, seed, d.stream);
+        d.sync();
+    });
+}
+
+namespace {
+
+// out[i] = op(a[i], b[i]) through the device field routines the kernels use;
+// every result canonical.
+__global__ void k_gl_ops(int op, const uint64_t *a, const uint64_t *b, uint64_t *out, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t x = a[i], y = b[i];
+    uint64_t r;
+    switch (op) {
+    case 0: r = gl::canon(gl::mul_lazy(x, y)); break;
+    case 1: r = gl::mul(x, y); break;
+    case 2: r = gl::canon(gl::reduce128_lazy(x, y)); break;
+    case 3: r = gl::canon(gl::reduce96_lazy(x, gl::lo32(y))); break;
+    case 4: r = gl::canon(gl::pow7_lazy(x)); break;
+    case 5: r = gl::add(x, y); break;
+    case 6: r = gl::sub(x, y); break;
+    default: r = gl::canon(x); break;
+    }
+    out[i] = r;
+}
+
+}  // namespace
+
+RustError zkx_gl_ops(size_t device_id, int op, const void *a, const void *b, void *out, size_t n)
+{
+    return guarded([&] {
+        Device &d = device(device_id);
+        if (op < 0 || op > 7) invalid("gl ops: op must be 0..7");
+        if (!n) return;
+        if (!a || !b || !out) invalid("null pointer");
+        if (n > (1ull << 31)) invalid("gl ops: n above 2^31");
+        const unsigned blocks = (unsigned)((n + 255) / 256);
+        hipLaunchKernelGGL(k_gl_ops, dim3(blocks), dim3(256), 0, d.stream, op, static_cast<const uint64_t *>(a),
+                           static_cast<const uint64_t *>(b), static_cast<uint64_t *>(out), (uint64_t)n);
+        ZKX_HIP(hipGetLastError());
         d.sync();
     });
 }

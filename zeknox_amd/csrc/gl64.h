@@ -105,18 +105,30 @@ __device__ __forceinline__ uint64_t reduce128(uint64_t lo, uint64_t hi)
 // Canonical residue of top * 2^64 + lo with top < 2^32.
 __device__ __forceinline__ uint64_t reduce96(uint64_t lo, uint32_t top) { return add(lo, mul_eps(top)); }
 
+// ---- carry-out forms
+//
+// v_mad_u64_u32 writes bit 64 of S0 * S1 + S2 to an SGPR lane mask; the
+// compiler does not expose that carry, so the lazy reductions below reach it
+// through inline asm.  gfx950 needs two wait states between a VALU write of
+// a lane mask and a VALU read of it (the compiler inserts the same for its
+// own carry chains): every mask is created and consumed inside one asm
+// block, behind an s_nop 1.  The inline constant -1 is 0xFFFFFFFF = EPS as
+// the 32-bit multiplicand.
+
 // Lazy residue (below 2^64, congruent mod p) of top * 2^64 + lo, top < 2^32:
-// y = lo + top * EPS (< 2^65, carry c), then y + c * EPS = y + c * 2^32 - c,
-// which cannot carry again (c leaves y <= 2^64 - 2^33).
+// u = lo + top * EPS (mod 2^64, carry c: one v_mad_u64_u32), then
+// u + c * EPS, which cannot carry again (c leaves u <= 2^64 - 2^33).
 __device__ __forceinline__ uint64_t reduce96_lazy(uint64_t lo, uint32_t top)
 {
-    const uint64_t m = (uint64_t)top * opaque(0xFFFFFFFFu);
-    uint32_t c1, c, k, k2, k3;
-    const uint32_t ylo = __builtin_addc(lo32(lo), lo32(m), 0u, &c1);
-    const uint32_t yhi = __builtin_addc(hi32(lo), hi32(m), c1, &c);
-    const uint32_t rlo = __builtin_subc(ylo, 0u, c, &k);
-    const uint32_t rhi = __builtin_subc(__builtin_addc(yhi, 0u, c, &k2), 0u, k, &k3);
-    return pack(rlo, rhi);
+    uint64_t u, r, s0, s1;
+    uint32_t cv;
+    asm("v_mad_u64_u32 %[u], %[s0], %[top], -1, %[lo]\n\t"
+        "s_nop 1\n\t"
+        "v_cndmask_b32_e64 %[cv], 0, 1, %[s0]\n\t"
+        "v_mad_u64_u32 %[r], %[s1], %[cv], -1, %[u]"
+        : [u] "=&v"(u), [r] "=&v"(r), [s0] "=&s"(s0), [s1] "=&s"(s1), [cv] "=&v"(cv)
+        : [top] "v"(top), [lo] "v"(lo));
+    return r;
 }
 
 // 64x64 -> 128 product from five 32x32+64 multiply-adds (any a, b < 2^64).
@@ -135,27 +147,42 @@ __device__ __forceinline__ void mul128(uint64_t a, uint64_t b, uint64_t &lo, uin
 
 // A residue of hi * 2^64 + lo (any 128-bit value) below 2^64 but not
 // necessarily below p ("lazy"):  hi * 2^64 = h0 * EPS - h1 (mod p).
-//   y = lo + h0 * EPS       (< 2^65; carry c)
-//   z = y - h1 (mod 2^64)   (borrow b)
+//   u = lo + h0 * EPS (mod 2^64; carry c)      one v_mad_u64_u32
+//   z = u - h1 (mod 2^64; borrow b)
 //   r = z + (c - b) * EPS   -- each case lands in [0, 2^64): c without b
 //       leaves z <= 2^64 - 2^33, b without c leaves z >= 2^64 - 2^32 + 1,
-//       so the wrapped 64-bit sum is the exact value.
+//       so the wrapped 64-bit sum is the exact value.  With cb = c - b in
+//       {-1, 0, 1}: z + cb * EPS = (z - cb) + cb * 2^32, i.e. one signed
+//       v_mad_i64_i32 (cb * -1 + z) and a 32-bit add to the high word.
 __device__ __forceinline__ uint64_t reduce128_lazy(uint64_t lo, uint64_t hi)
 {
     const uint32_t h0 = lo32(hi), h1 = hi32(hi);
-    const uint64_t m = (uint64_t)h0 * opaque(0xFFFFFFFFu);
-    uint32_t c1, c, b1, b;
-    const uint32_t ylo = __builtin_addc(lo32(lo), lo32(m), 0u, &c1);
-    const uint32_t yhi = __builtin_addc(hi32(lo), hi32(m), c1, &c);
-    const uint32_t zlo = __builtin_subc(ylo, h1, 0u, &b1);
-    const uint32_t zhi = __builtin_subc(yhi, 0u, b1, &b);
-    // (c - b) * EPS as a 64-bit two's-complement addend: +EPS, -EPS or 0
-    const uint32_t dlo = c > b ? 0xFFFFFFFFu : (b > c ? 1u : 0u);
-    const uint32_t dhi = b > c ? 0xFFFFFFFFu : 0u;
-    uint32_t k1, k2;
-    const uint32_t rlo = __builtin_addc(zlo, dlo, 0u, &k1);
-    const uint32_t rhi = __builtin_addc(zhi, dhi, k1, &k2);
-    return pack(rlo, rhi);
+    uint64_t u, sc;
+    uint32_t cv;
+    asm("v_mad_u64_u32 %[u], %[sc], %[h0], -1, %[lo]\n\t"
+        "s_nop 1\n\t"
+        "v_cndmask_b32_e64 %[cv], 0, 1, %[sc]"
+        : [u] "=&v"(u), [sc] "=&s"(sc), [cv] "=&v"(cv)
+        : [h0] "v"(h0), [lo] "v"(lo));
+    uint32_t zlo, zhi, cb;
+    uint64_t b1, b, bj;
+    asm("v_sub_co_u32_e64 %[zlo], %[b1], %[ulo], %[h1]\n\t"
+        "s_nop 1\n\t"
+        "v_subb_co_u32_e64 %[zhi], %[b], %[uhi], 0, %[b1]\n\t"
+        "s_nop 1\n\t"
+        "v_subb_co_u32_e64 %[cb], %[bj], %[cv], 0, %[b]"
+        : [zlo] "=&v"(zlo), [zhi] "=&v"(zhi), [cb] "=&v"(cb), [b1] "=&s"(b1), [b] "=&s"(b), [bj] "=&s"(bj)
+        : [ulo] "v"(lo32(u)), [uhi] "v"(hi32(u)), [h1] "v"(h1), [cv] "v"(cv));
+    uint64_t t, tj;
+    asm("v_mad_i64_i32 %[t], %[tj], %[cb], -1, %[z]"
+        : [t] "=&v"(t), [tj] "=&s"(tj)
+        : [cb] "v"(cb), [z] "v"(pack(zlo, zhi)));
+    // The high-word add stays in asm: written in C, the compiler folds it
+    // into a following carry chain (canon's hi + 0 + carry becomes
+    // hi(t) + cb + carry), whose carry-out then counts the wrap of this add.
+    uint32_t rhi;
+    asm("v_add_u32 %[r], %[a], %[b]" : [r] "=v"(rhi) : [a] "v"(hi32(t)), [b] "v"(cb));
+    return pack(lo32(t), rhi);
 }
 
 // x mod p for any x < 2^64 (x >= p iff x + EPS carries).
