@@ -32,6 +32,13 @@ __device__ __forceinline__ uint64_t tw_at(const uint64_t *tw, uint32_t lg_tw, ui
     return tw[inverse ? ((0 - k) & mask) : k];
 }
 
+// Stage twiddle of a contiguous pass: w_4096^k (w^-k for the inverse),
+// k < 2048, read from the resident 4096-entry table (16 KB of it hot in L1).
+__device__ __forceinline__ uint64_t tw12_at(const NttPass &a, uint32_t k)
+{
+    return a.tw12[a.inverse ? ((0u - k) & ((1u << kNttTileLog) - 1)) : k];
+}
+
 __device__ __forceinline__ uint32_t brev_bits(uint32_t x, uint32_t bits)
 {
     return bits ? (__builtin_bitreverse32(x) >> (32 - bits)) : 0u;
@@ -60,8 +67,8 @@ __device__ __forceinline__ uint64_t tile_index(const NttPass &a, uint64_t tile, 
 // from the lane id.  The twiddle of stage d for element e depends only on
 // (j mod 2^d) << s_lo | lo.
 template <bool CONTIG, int K>
-__device__ __forceinline__ void ntt_round(uint64_t *lds, const uint64_t *twl, const NttPass &a, uint32_t t,
-                                          uint32_t b0, uint32_t jpos, uint64_t lo_base)
+__device__ __forceinline__ void ntt_round(uint64_t *lds, const NttPass &a, uint32_t t, uint32_t b0, uint32_t jpos,
+                                          uint64_t lo_base)
 {
     constexpr int E = kNttPerThread;
     const uint32_t RB = jpos + b0;
@@ -87,7 +94,7 @@ __device__ __forceinline__ void ntt_round(uint64_t *lds, const uint64_t *twl, co
             const uint32_t j = !CONTIG ? (u >> a.c) : (u & ((1u << a.m) - 1));
             uint64_t w;
             if constexpr (CONTIG) {
-                w = twl[(j & ((1u << d) - 1)) << (kNttTileLog - 1 - d)];
+                w = tw12_at(a, (j & ((1u << d) - 1)) << (kNttTileLog - 1 - d));
             } else {
                 const uint64_t lo_part = lo_base + (u & ((1u << a.c) - 1));
                 const uint64_t ex = ((((uint64_t)(j & ((1u << d) - 1))) << a.s_lo) | lo_part) << shift;
@@ -105,8 +112,8 @@ __device__ __forceinline__ void ntt_round(uint64_t *lds, const uint64_t *twl, co
 // Full-width round (K = 4, no extra element bits): the lane's other u-bits
 // are its id, so the 15 distinct twiddles of the round are loaded once.
 template <bool CONTIG>
-__device__ __forceinline__ void ntt_round4(uint64_t *lds, const uint64_t *twl, const NttPass &a, uint32_t t,
-                                           uint32_t b0, uint32_t jpos, uint64_t lo_base)
+__device__ __forceinline__ void ntt_round4(uint64_t *lds, const NttPass &a, uint32_t t, uint32_t b0, uint32_t jpos,
+                                           uint64_t lo_base)
 {
     constexpr int E = kNttPerThread;
     const uint32_t RB = jpos + b0;
@@ -124,9 +131,9 @@ __device__ __forceinline__ void ntt_round4(uint64_t *lds, const uint64_t *twl, c
         const uint32_t shift = a.lg_tw - (a.s_lo + d) - 1;
         uint64_t w[1 << dd];
         if constexpr (CONTIG) {
-            // contiguous pass: w_{2^(d+1)}^x = w_4096^(x << (11 - d)) from LDS
+            // contiguous pass: w_{2^(d+1)}^x = w_4096^(x << (11 - d))
 #pragma unroll
-            for (int v = 0; v < (1 << dd); v++) w[v] = twl[(j0 | ((uint32_t)v << b0)) << (kNttTileLog - 1 - d)];
+            for (int v = 0; v < (1 << dd); v++) w[v] = tw12_at(a, (j0 | ((uint32_t)v << b0)) << (kNttTileLog - 1 - d));
         } else {
 #pragma unroll
             for (int v = 0; v < (1 << dd); v++)
@@ -148,12 +155,12 @@ __device__ __forceinline__ void ntt_round4(uint64_t *lds, const uint64_t *twl, c
 // `passes` (npasses > 0, device memory) lists independent passes sorted by
 // tile_begin; with npasses == 0 the launch runs the single pass `one`.
 // CONTIG: every pass of the launch is contiguous (s_lo == 0), else strided.
+// Four work-groups per CU: 33 KB of LDS and at most 128 VGPRs each.
 template <bool CONTIG>
-__global__ void __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_eu(3))) k_ntt_tiles(const NttPass *__restrict__ passes, uint32_t npasses,
-                                                         NttPass one)
+__global__ void __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_eu(4)))
+k_ntt_tiles(const NttPass *__restrict__ passes, uint32_t npasses, NttPass one)
 {
     __shared__ uint64_t lds[(1 << kNttTileLog) + (1 << kNttTileLog) / 32];
-    __shared__ uint64_t twl[1 << (kNttTileLog - 1)];  // w_4096^k (or ^-k), k < 2048
 
     // locate this work-group's pass (uniform)
     uint32_t lo = 0, hi = npasses;
@@ -187,14 +194,6 @@ __global__ void __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_e
         }
         lds[lds_slot(u)] = v;
     }
-    // contiguous passes take every stage twiddle from one LDS table
-    if (CONTIG && a.m > 0) {
-#pragma unroll
-        for (int i = 0; i < (1 << (kNttTileLog - 1)) / kNttThreads; i++) {
-            const uint32_t k = t + kNttThreads * i;
-            twl[k] = a.tw12[a.inverse ? ((0u - k) & ((1u << kNttTileLog) - 1)) : k];
-        }
-    }
     __syncthreads();
 
     // low in-block bits shared by the tile's columns (strided passes)
@@ -205,12 +204,12 @@ __global__ void __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_e
     uint32_t rem = a.m;
     while (rem >= 4) {
         rem -= 4;
-        ntt_round4<CONTIG>(lds, twl, a, t, rem, jpos, lo_base);
+        ntt_round4<CONTIG>(lds, a, t, rem, jpos, lo_base);
         __syncthreads();
     }
-    if (rem == 3) ntt_round<CONTIG, 3>(lds, twl, a, t, 0, jpos, lo_base);
-    else if (rem == 2) ntt_round<CONTIG, 2>(lds, twl, a, t, 0, jpos, lo_base);
-    else if (rem == 1) ntt_round<CONTIG, 1>(lds, twl, a, t, 0, jpos, lo_base);
+    if (rem == 3) ntt_round<CONTIG, 3>(lds, a, t, 0, jpos, lo_base);
+    else if (rem == 2) ntt_round<CONTIG, 2>(lds, a, t, 0, jpos, lo_base);
+    else if (rem == 1) ntt_round<CONTIG, 1>(lds, a, t, 0, jpos, lo_base);
     if (rem) __syncthreads();
 
     // ---- store, coalesced
