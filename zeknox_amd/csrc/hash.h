@@ -80,14 +80,27 @@ __device__ __forceinline__ void poseidon_permute(uint64_t s[12])
 {
 #pragma unroll
     for (int i = 0; i < 12; i++) s[i] = gl::add(s[i], c_poseidon_rc[i]);
-    // rounds 0-3 and 26-29 are full (S-box on all 12), 4-25 partial (s0 only)
+    // rounds 0-3 and 26-29 are full (S-box on all 12), 4-25 partial (s0
+    // only); the partial rounds run two per iteration, so each round's MDS
+    // writes where the next reads (no loop-carried register copies)
 #pragma unroll 1
-    for (int r = 0; r < 30; r++) {
-        s[0] = gl::pow7_lazy(s[0]);
-        if (r < 4 || r >= 26) {
+    for (int r = 0; r < 4; r++) {
 #pragma unroll
-            for (int i = 1; i < 12; i++) s[i] = gl::pow7_lazy(s[i]);
+        for (int i = 0; i < 12; i++) s[i] = gl::pow7_lazy(s[i]);
+        poseidon_mds(s, c_poseidon_rc + 12 * (r + 1));
+    }
+#pragma unroll 1
+    for (int r = 4; r < 26; r += 2) {
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            s[0] = gl::pow7_lazy(s[0]);
+            poseidon_mds(s, c_poseidon_rc + 12 * (r + h + 1));
         }
+    }
+#pragma unroll 1
+    for (int r = 26; r < 30; r++) {
+#pragma unroll
+        for (int i = 0; i < 12; i++) s[i] = gl::pow7_lazy(s[i]);
         poseidon_mds(s, c_poseidon_rc + 12 * (r + 1));
     }
 }
