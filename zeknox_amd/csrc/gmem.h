@@ -1,0 +1,24 @@
+// Global-memory views of device pointers.
+//
+// Pointers that a kernel reads out of a descriptor in memory are generic to
+// the compiler, which then emits flat loads and stores.  A flat access counts
+// against both the vector-memory and the LDS/scalar-memory wait counters, so
+// every wait for an LDS access or a scalar constant load (the round constants
+// of the hash permutations) also waits for the HBM accesses in flight.
+// Viewed through the global address space the same accesses become global_*
+// instructions, which count against the vector-memory counter only.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace zkx {
+
+template <class T>
+using gptr = __attribute__((address_space(1))) T *;
+
+template <class T>
+__device__ __forceinline__ gptr<T> gmem(T *p)
+{
+    return (gptr<T>)p;
+}
+
+}  // namespace zkx

@@ -33,6 +33,7 @@
 // work-group b runs tile b - tile_begin of the job with the largest
 // tile_begin <= b.
 #pragma once
+#include "gmem.h"
 #include "hash.h"
 #include "merkle_types.h"
 
@@ -45,9 +46,9 @@ namespace zkx {
 constexpr int merkle_waves_per_eu(int ht) { return (ht == kPoseidon || ht == kPoseidon2) ? 6 : 1; }
 
 // Digest slot of node `idx` of the level of width 2^lg_w in subtree `sub`.
-__device__ __forceinline__ uint64_t *node_ptr(const MerkleArgs &a, uint64_t sub, uint32_t lg_w, uint64_t idx)
+__device__ __forceinline__ gptr<uint64_t> node_ptr(const MerkleArgs &a, uint64_t sub, uint32_t lg_w, uint64_t idx)
 {
-    return a.dig + 4 * (sub * a.sdl + ((1ull << lg_w) - 2) + idx);
+    return gmem(a.dig) + 4 * (sub * a.sdl + ((1ull << lg_w) - 2) + idx);
 }
 
 __device__ __forceinline__ const MerkleArgs *find_job(const MerkleArgs *jobs, uint32_t njobs)
@@ -79,9 +80,9 @@ k_merkle_tile(const MerkleArgs *__restrict__ jobs, uint32_t njobs, MerkleArgs on
     if (t < cur) {
         const uint64_t g = first + t;
         const uint64_t sub = g >> lg_w, idx = g & ((1ull << lg_w) - 1);
-        const uint64_t *row = a.leaves + g * a.leaf_stride;
+        const gptr<const uint64_t> row = gmem(a.leaves) + g * a.leaf_stride;
         const uint64_t cs = a.col_stride;
-        uint64_t *out = a.leaves_out ? a.leaves_out + g * a.leaf_size : nullptr;
+        const gptr<uint64_t> out = a.leaves_out ? gmem(a.leaves_out) + g * a.leaf_size : nullptr;
         hash_one<HT>(a.leaf_size,
                      [&](uint32_t k) {
                          const uint64_t v = row[(uint64_t)k * cs];
@@ -89,7 +90,7 @@ k_merkle_tile(const MerkleArgs *__restrict__ jobs, uint32_t njobs, MerkleArgs on
                          return v;
                      },
                      h);
-        uint64_t *dst = lg_w ? node_ptr(a, sub, lg_w, idx) : a.cap + 4 * g;
+        const gptr<uint64_t> dst = lg_w ? node_ptr(a, sub, lg_w, idx) : gmem(a.cap) + 4 * g;
 #pragma unroll
         for (int i = 0; i < 4; i++) dst[i] = h[i];
 #pragma unroll
@@ -130,7 +131,7 @@ k_merkle_tile(const MerkleArgs *__restrict__ jobs, uint32_t njobs, MerkleArgs on
             first = grp << tl;
             if (t < cur) {
                 const uint64_t g = first + t;
-                const uint64_t *src = node_ptr(a, g >> lg_w, lg_w, g & ((1ull << lg_w) - 1));
+                const gptr<uint64_t> src = node_ptr(a, g >> lg_w, lg_w, g & ((1ull << lg_w) - 1));
 #pragma unroll
                 for (int i = 0; i < 4; i++) lds[t][i] = src[i];
             }
@@ -145,7 +146,7 @@ k_merkle_tile(const MerkleArgs *__restrict__ jobs, uint32_t njobs, MerkleArgs on
                 hash_two<HT>(lds[2 * t], lds[2 * t + 1], out);
                 const uint64_t sub = (first >> 1) + t;
 #pragma unroll
-                for (int i = 0; i < 4; i++) a.cap[4 * sub + i] = out[i];
+                for (int i = 0; i < 4; i++) gmem(a.cap)[4 * sub + i] = out[i];
             }
             return;
         }
@@ -158,7 +159,7 @@ k_merkle_tile(const MerkleArgs *__restrict__ jobs, uint32_t njobs, MerkleArgs on
         if (t < half) {
             const uint64_t g = first + t;
             const uint64_t sub = g >> lg_w, idx = g & ((1ull << lg_w) - 1);
-            uint64_t *dst = node_ptr(a, sub, lg_w, idx);
+            const gptr<uint64_t> dst = node_ptr(a, sub, lg_w, idx);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 dst[i] = out[i];

@@ -22,6 +22,7 @@
 // the zero half is never read, written or butterflied.
 #pragma once
 #include "gl64.h"
+#include "gmem.h"
 #include "ntt_types.h"
 
 namespace zkx {
@@ -29,14 +30,14 @@ namespace zkx {
 __device__ __forceinline__ uint64_t tw_at(const uint64_t *tw, uint32_t lg_tw, uint64_t k, bool inverse)
 {
     const uint64_t mask = (1ull << lg_tw) - 1;
-    return tw[inverse ? ((0 - k) & mask) : k];
+    return gmem(tw)[inverse ? ((0 - k) & mask) : k];
 }
 
 // Stage twiddle of a contiguous pass: w_4096^k (w^-k for the inverse),
 // k < 2048, read from the resident 4096-entry table (16 KB of it hot in L1).
 __device__ __forceinline__ uint64_t tw12_at(const NttPass &a, uint32_t k)
 {
-    return a.tw12[a.inverse ? ((0u - k) & ((1u << kNttTileLog) - 1)) : k];
+    return gmem(a.tw12)[a.inverse ? ((0u - k) & ((1u << kNttTileLog) - 1)) : k];
 }
 
 __device__ __forceinline__ uint32_t brev_bits(uint32_t x, uint32_t bits)
@@ -173,9 +174,8 @@ k_ntt_tiles(const NttPass *__restrict__ passes, uint32_t npasses, NttPass one)
     const uint64_t tile = blockIdx.x - a.tile_begin;
     const uint32_t t = threadIdx.x;
 
-    // ---- load, coalesced: lane t takes positions t + 256 i (4 at a time in
-    // flight keeps the address and data registers below the round's needs)
-#pragma unroll 4
+    // ---- load, coalesced: lane t takes positions t + 256 i, all 16 in flight
+#pragma unroll
     for (int i = 0; i < kNttPerThread; i++) {
         const uint32_t u = t + kNttThreads * i;
         const uint64_t idx = tile_index(a, tile, u);
@@ -186,10 +186,10 @@ k_ntt_tiles(const NttPass *__restrict__ passes, uint32_t npasses, NttPass one)
                 // column col holds a_i * F[b*n + i], F the resident factor table
                 const uint64_t n = 1ull << a.L;
                 const uint64_t col = idx >> (a.L + a.rate), pos = idx & (n - 1);
-                v = gl::mul(a.src[(col << a.L) + pos], a.coset[idx & ((n << a.rate) - 1)]);
+                v = gl::mul(gmem(a.src)[(col << a.L) + pos], gmem(a.coset)[idx & ((n << a.rate) - 1)]);
             } else {
-                v = a.buf[idx];
-                if (a.mode == kSrcCoset) v = gl::mul(v, a.coset[idx & ((1ull << a.L) - 1)]);
+                v = gmem(a.buf)[idx];
+                if (a.mode == kSrcCoset) v = gl::mul(v, gmem(a.coset)[idx & ((1ull << a.L) - 1)]);
             }
         }
         lds[lds_slot(u)] = v;
@@ -217,7 +217,7 @@ k_ntt_tiles(const NttPass *__restrict__ passes, uint32_t npasses, NttPass one)
     for (int i = 0; i < kNttPerThread; i++) {
         const uint32_t u = t + kNttThreads * i;
         const uint64_t idx = tile_index(a, tile, u);
-        if (idx < a.total) a.buf[idx] = lds[lds_slot(u)];
+        if (idx < a.total) gmem(a.buf)[idx] = lds[lds_slot(u)];
     }
 }
 
@@ -240,7 +240,7 @@ struct BitrevArgs {
 __device__ __forceinline__ uint64_t br_post(const BitrevArgs &a, uint64_t v, uint64_t t)
 {
     if (a.use_scale) v = gl::mul(v, a.scale);
-    if (a.use_coset) v = gl::mul(v, a.coset[t]);
+    if (a.use_coset) v = gl::mul(v, gmem(a.coset)[t]);
     return v;
 }
 
@@ -255,7 +255,7 @@ __global__ void __launch_bounds__(kNttThreads) k_bitrev_tiles(BitrevArgs a)
     const uint32_t mid = blockIdx.x & ((1u << mid_bits) - 1);
     const uint32_t rmid = brev_bits(mid, mid_bits);
     if (rmid < mid) return;  // the partner tile's block does both
-    uint64_t *base = a.buf + col * n;
+    const gptr<uint64_t> base = gmem(a.buf) + col * n;
     const uint32_t hs = a.lg - kBrTile;
 
     for (uint32_t e = threadIdx.x; e < T * T; e += kNttThreads) {
@@ -286,7 +286,7 @@ __global__ void __launch_bounds__(kNttThreads) k_bitrev_small(BitrevArgs a)
     const uint64_t col = g >> a.lg, i = g & (n - 1);
     const uint64_t r = brev_bits((uint32_t)i, a.lg);
     if (r < i) return;
-    uint64_t *base = a.buf + col * n;
+    const gptr<uint64_t> base = gmem(a.buf) + col * n;
     const uint64_t x = base[i], y = base[r];
     base[i] = br_post(a, y, i);
     if (r != i) base[r] = br_post(a, x, r);
