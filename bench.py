@@ -64,7 +64,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--jobs", type=int, default=4, help="135 x 2^15 rate-3 commits per batched step")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
-    ap.add_argument("--serve-seconds", type=float, default=10.0, help="served-stream duration (0: skip)")
+    ap.add_argument("--serve-seconds", type=float, default=60.0, help="served-stream duration (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=40.0, help="served-stream arrivals per second")
     ap.add_argument("--serve-trace", default=None,
                     help="write one CSV line per served request (times in ms from the first arrival)")
