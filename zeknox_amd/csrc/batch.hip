@@ -115,13 +115,17 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
     // synchronised before first use)
     std::vector<const uint64_t *> tw(n), F(n);
     const uint64_t *tw12 = nullptr;
+    const uint64_t *const *tw_lg = nullptr;
     {
         std::lock_guard<std::mutex> g(dev_.mu);
-        bool built = dev_.twiddles[kNttTileLog] == nullptr;
+        bool built = dev_.twiddles[kNttTileLog] == nullptr || !dev_.twiddle_ptrs.p;
         tw12 = dev_.twiddle_table(kNttTileLog);
+        uint32_t lg_max = 0;
+        for (size_t i = 0; i < n; i++) lg_max = std::max(lg_max, jobs[i].lg_n + jobs[i].rate);
+        for (uint32_t k = 1; k <= lg_max; k++) built |= dev_.twiddles[k] == nullptr;
+        tw_lg = dev_.twiddle_tables_upto(lg_max);
         for (size_t i = 0; i < n; i++) {
             const uint32_t lg = jobs[i].lg_n + jobs[i].rate;
-            built |= dev_.twiddles[lg] == nullptr;
             tw[i] = dev_.twiddle_table(lg);
             const size_t before = dev_.lde_factors.size();
             F[i] = lde_factor_table(dev_, jobs[i].lg_n, jobs[i].rate, false, true);
@@ -140,7 +144,7 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
     for (size_t i = 0; i < n; i++) {
         const CommitJob &j = jobs[i];
         const uint32_t lg = j.lg_n + j.rate;
-        ntt_tiles[i] = plan_dif(ntt_plan[i], lde[i], (uint64_t)j.cols << j.rate, j.lg_n, lg, tw[i], tw12, false,
+        ntt_tiles[i] = plan_dif(ntt_plan[i], lde[i], (uint64_t)j.cols << j.rate, j.lg_n, lg, tw[i], tw12, tw_lg, false,
                                 kSrcLde, j.coeffs, j.rate, false, F[i], tun[i]);
         MerkleSpec s;
         s.digests = j.digests;

@@ -30,9 +30,12 @@ void transpose_rev(Device &d, uint64_t *out, const uint64_t *in, uint32_t lg, ui
 // Pass descriptors of the DIF over nblk blocks of 2^L in buf (in launch
 // order; each pass must finish before the next starts), cut into passes as
 // `tun` says.  Returns the tile count of every pass.
+// tw_lg: the device array of twiddle tables by log size, built up to L
+// (Device::twiddle_tables_upto).
 uint64_t plan_dif(std::vector<NttPass> &out, uint64_t *buf, uint64_t nblk, uint32_t L, uint32_t lg_tw,
-                  const uint64_t *tw, const uint64_t *tw12, bool inverse, uint32_t mode0, const uint64_t *src,
-                  uint32_t rate, bool with_coset, const uint64_t *coset, const Tuning &tun);
+                  const uint64_t *tw, const uint64_t *tw12, const uint64_t *const *tw_lg, bool inverse,
+                  uint32_t mode0, const uint64_t *src, uint32_t rate, bool with_coset, const uint64_t *coset,
+                  const Tuning &tun);
 
 // Stage counts of the DIF passes over blocks of 2^L under `tun`, top
 // stages first (the last pass contiguous).

@@ -34,8 +34,9 @@ void run_dif(Device &d, uint64_t *buf, uint64_t nblk, uint32_t L, uint32_t lg_tw
 {
     std::vector<NttPass> passes;
     const uint64_t *tw12 = d.twiddle_table(kNttTileLog);
-    uint64_t tiles = plan_dif(passes, buf, nblk, L, lg_tw, tw, tw12, inverse, mode0, src, rate, with_coset, coset,
-                              d.tuning_default());
+    const uint64_t *const *tw_lg = d.twiddle_tables_upto(L);
+    uint64_t tiles = plan_dif(passes, buf, nblk, L, lg_tw, tw, tw12, tw_lg, inverse, mode0, src, rate, with_coset,
+                              coset, d.tuning_default());
     for (const NttPass &p : passes) launch_pass(d, p, tiles, p.mode == kSrcLde ? "ntt_pass_lde" : "ntt_pass");
 }
 
@@ -96,8 +97,9 @@ std::vector<uint32_t> ntt_pass_sizes(uint32_t L, const Tuning &tun)
 }
 
 uint64_t plan_dif(std::vector<NttPass> &out, uint64_t *buf, uint64_t nblk, uint32_t L, uint32_t lg_tw,
-                  const uint64_t *tw, const uint64_t *tw12, bool inverse, uint32_t mode0, const uint64_t *src,
-                  uint32_t rate, bool with_coset, const uint64_t *coset, const Tuning &tun)
+                  const uint64_t *tw, const uint64_t *tw12, const uint64_t *const *tw_lg, bool inverse,
+                  uint32_t mode0, const uint64_t *src, uint32_t rate, bool with_coset, const uint64_t *coset,
+                  const Tuning &tun)
 {
     const uint64_t total = nblk << L;
     const uint64_t tiles = (total + (1ull << kNttTileLog) - 1) >> kNttTileLog;
@@ -110,6 +112,7 @@ uint64_t plan_dif(std::vector<NttPass> &out, uint64_t *buf, uint64_t nblk, uint3
         a.tw = tw;
         a.coset = coset;
         a.tw12 = tw12;
+        a.tw_lg = tw_lg;
         a.total = total;
         a.L = L;
         a.lg_tw = lg_tw;

@@ -40,6 +40,15 @@ __device__ __forceinline__ uint64_t tw12_at(const NttPass &a, uint32_t k)
     return gmem(a.tw12)[a.inverse ? ((0u - k) & ((1u << kNttTileLog) - 1)) : k];
 }
 
+// Twiddle table of a strided pass's stage d (butterfly span 2^(s_lo+d+1)):
+// w_{2^(s_lo+d+1)}^k sits at index k, so the 2^c consecutive columns of a
+// tile read consecutive words (rather than words 2^(lg_tw-s_lo-d-1) apart
+// in the largest table).
+__device__ __forceinline__ const uint64_t *stage_table(const NttPass &a, uint32_t lg)
+{
+    return gmem(a.tw_lg)[lg];
+}
+
 __device__ __forceinline__ uint32_t brev_bits(uint32_t x, uint32_t bits)
 {
     return bits ? (__builtin_bitreverse32(x) >> (32 - bits)) : 0u;
@@ -86,7 +95,8 @@ __device__ __forceinline__ void ntt_round(uint64_t *lds, const NttPass &a, uint3
 #pragma unroll
     for (int dd = K - 1; dd >= 0; dd--) {
         const uint32_t d = b0 + (uint32_t)dd;
-        const uint32_t shift = a.lg_tw - (a.s_lo + d) - 1;
+        const uint32_t lg = a.s_lo + d + 1;
+        const uint64_t *tab = !CONTIG ? stage_table(a, lg) : nullptr;
 #pragma unroll
         for (int e = 0; e < E; e++) {
             if ((e >> dd) & 1) continue;
@@ -98,8 +108,8 @@ __device__ __forceinline__ void ntt_round(uint64_t *lds, const NttPass &a, uint3
                 w = tw12_at(a, (j & ((1u << d) - 1)) << (kNttTileLog - 1 - d));
             } else {
                 const uint64_t lo_part = lo_base + (u & ((1u << a.c) - 1));
-                const uint64_t ex = ((((uint64_t)(j & ((1u << d) - 1))) << a.s_lo) | lo_part) << shift;
-                w = tw_at(a.tw, a.lg_tw, ex, a.inverse);
+                const uint64_t ex = (((uint64_t)(j & ((1u << d) - 1))) << a.s_lo) | lo_part;
+                w = tw_at(tab, lg, ex, a.inverse);
             }
             const uint64_t y0 = x[e], y1 = x[f];
             x[e] = gl::add(y0, y1);
@@ -129,16 +139,16 @@ __device__ __forceinline__ void ntt_round4(uint64_t *lds, const NttPass &a, uint
 #pragma unroll
     for (int dd = 3; dd >= 0; dd--) {
         const uint32_t d = b0 + (uint32_t)dd;
-        const uint32_t shift = a.lg_tw - (a.s_lo + d) - 1;
+        const uint32_t lg = a.s_lo + d + 1;
         uint64_t w[1 << dd];
         if constexpr (CONTIG) {
             // contiguous pass: w_{2^(d+1)}^x = w_4096^(x << (11 - d))
 #pragma unroll
             for (int v = 0; v < (1 << dd); v++) w[v] = tw12_at(a, (j0 | ((uint32_t)v << b0)) << (kNttTileLog - 1 - d));
         } else {
+            const uint64_t *tab = stage_table(a, lg);
 #pragma unroll
-            for (int v = 0; v < (1 << dd); v++)
-                w[v] = tw_at(a.tw, a.lg_tw, (base + ((uint64_t)v << (b0 + a.s_lo))) << shift, a.inverse);
+            for (int v = 0; v < (1 << dd); v++) w[v] = tw_at(tab, lg, base + ((uint64_t)v << (b0 + a.s_lo)), a.inverse);
         }
 #pragma unroll
         for (int e = 0; e < E; e++) {

@@ -36,7 +36,9 @@ struct NttPass {
     uint32_t mode;          // SrcMode of the tile load
     uint32_t inverse;       // use w^-1
     uint32_t with_coset;    // unused by the kernel (the factor table carries the coset)
-    uint32_t pad_;
+    // device array: tw_lg[k][i] = w_{2^k}^i (strided stages: table of the
+    // stage's butterfly span, so consecutive columns read consecutive words)
+    const uint64_t *const *tw_lg;
 };
 
 }  // namespace zkx

@@ -69,6 +69,9 @@ struct Device {
 
     // Resident tables (twiddles per log size, coset powers).
     uint64_t *twiddles[33] = {};   // twiddles[k][i] = w_{2^k}^i, i < 2^k
+    // The same 33 table pointers in device memory (null where not built),
+    // for kernels that pick a table by log size.
+    DevBuf twiddle_ptrs;
     uint64_t *coset = nullptr;     // coset[i] = g^i, i < 2^coset_lg
     uint32_t coset_lg = 0;
     uint64_t coset_gen = 0;
@@ -85,6 +88,9 @@ struct Device {
     struct MerkleArgs *merkle_descriptor();
 
     const uint64_t *twiddle_table(uint32_t lg);  // builds on first use
+    // Builds the tables of every log size 1..lg; returns the device array of
+    // table pointers.
+    const uint64_t *const *twiddle_tables_upto(uint32_t lg);
     void build_coset(uint32_t lg, uint64_t gen);
     void sync();
 

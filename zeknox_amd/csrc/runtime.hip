@@ -121,8 +121,21 @@ const uint64_t *Device::twiddle_table(uint32_t lg)
         ZKX_HIP(hipMalloc(&p, n * sizeof(uint64_t)));
         fill_powers(p, n, omega(lg), stream);
         twiddles[lg] = p;
+        if (twiddle_ptrs.p)
+            ZKX_HIP(hipMemcpyAsync(twiddle_ptrs.as<uint64_t *>() + lg, &twiddles[lg], sizeof(uint64_t *),
+                                   hipMemcpyHostToDevice, stream));
     }
     return twiddles[lg];
+}
+
+const uint64_t *const *Device::twiddle_tables_upto(uint32_t lg)
+{
+    if (!twiddle_ptrs.p) {
+        twiddle_ptrs = DevBuf(sizeof(twiddles));
+        ZKX_HIP(hipMemcpyAsync(twiddle_ptrs.p, twiddles, sizeof(twiddles), hipMemcpyHostToDevice, stream));
+    }
+    for (uint32_t k = 1; k <= lg; k++) twiddle_table(k);
+    return twiddle_ptrs.as<const uint64_t *const>();
 }
 
 void Device::build_coset(uint32_t lg, uint64_t gen)
