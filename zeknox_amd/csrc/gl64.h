@@ -208,6 +208,58 @@ __device__ __forceinline__ uint64_t mul(uint64_t a, uint64_t b) { return canon(m
 
 __device__ __forceinline__ uint64_t sqr(uint64_t a) { return mul(a, a); }
 
+// ---- multiplication by a power of two known at compile time
+//
+// 2 has order 192 mod p (2^96 = -1), and the 2^k-th roots of unity of the
+// NTT for k <= 4 are powers of two (OMEGA[1..4] = 2^96, 2^48, 2^120, 2^156),
+// so a butterfly whose twiddle is such a root multiplies by shifting.
+
+constexpr uint64_t cmul(uint64_t a, uint64_t b) { return (uint64_t)((unsigned __int128)a * b % P); }
+constexpr uint64_t cpow(uint64_t b, uint64_t e)
+{
+    uint64_t r = 1;
+    for (; e; e >>= 1, b = cmul(b, b))
+        if (e & 1) r = cmul(r, b);
+    return r;
+}
+// OMEGA[k] = OMEGA[32]^(2^(32-k)) (gl64_params.hpp:48 gives OMEGA[32])
+constexpr uint64_t kOmega32 = 1753635133440165772ull;
+constexpr uint64_t comega(int k) { return cpow(kOmega32, 1ull << (32 - k)); }
+// e with 2^e = w (mod p), -1 if w is not a power of two
+constexpr int clog2(uint64_t w)
+{
+    for (int e = 0; e < 192; e++)
+        if (cpow(2, e) == w) return e;
+    return -1;
+}
+
+// x * 2^T, lazy, for 0 <= T < 32 and any x < 2^64: the 96-bit value
+// (x << T) + (x >> (64 - T)) * 2^64.
+template <int T>
+__device__ __forceinline__ uint64_t shl_lazy(uint64_t x)
+{
+    static_assert(T >= 0 && T < 32, "shift below 32");
+    if constexpr (T == 0) return x;
+    else return reduce96_lazy(x << T, (uint32_t)(x >> (64 - T)));
+}
+
+// x * 2^32, lazy: lo(x) * 2^32 + hi(x) * 2^64 = pack(0, lo(x)) + hi(x) * EPS.
+__device__ __forceinline__ uint64_t shl32_lazy(uint64_t x) { return reduce96_lazy(pack(0u, lo32(x)), hi32(x)); }
+
+// x * 2^E mod p, canonical, for 0 <= E < 192 and any x < 2^64.
+template <int E>
+__device__ __forceinline__ uint64_t mul_pow2(uint64_t x)
+{
+    static_assert(E >= 0 && E < 192, "exponent below the order of 2");
+    constexpr int t = E % 96;
+    uint64_t r = shl_lazy<t % 32>(x);
+    if constexpr (t >= 32) r = shl32_lazy(r);
+    if constexpr (t >= 64) r = shl32_lazy(r);
+    r = canon(r);
+    if constexpr (E >= 96) r = neg(r);
+    return r;
+}
+
 // x^7 with the addition chain x2 = x^2, x3 = x*x2, x4 = x2^2, x7 = x3*x4;
 // lazy in and out (any x < 2^64; result below 2^64, congruent mod p).
 __device__ __forceinline__ uint64_t pow7_lazy(uint64_t x)

@@ -21,6 +21,8 @@
 // coefficients directly and multiplies by the resident factor table F, so
 // the zero half is never read, written or butterflied.
 #pragma once
+#include <type_traits>
+
 #include "gl64.h"
 #include "gmem.h"
 #include "ntt_types.h"
@@ -163,6 +165,76 @@ __device__ __forceinline__ void ntt_round4(uint64_t *lds, const NttPass &a, uint
     for (int e = 0; e < E; e++) lds[lds_slot(ubase | ((uint32_t)e << RB))] = x[e];
 }
 
+// ---- last register round of a contiguous pass
+//
+// With b0 = 0 the round covers the K lowest stages; stage d's twiddle for
+// element e is w_{2^(d+1)}^v with v = e mod 2^d, a root of order <= 16:
+// w = 1 (no product) or a power of two (a shift, mul_pow2).
+
+template <int I, int N, class F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
+// x * w_{2^(D+1)}^V (w^-V for the inverse), x canonical
+template <int D, int V, bool INV>
+__device__ __forceinline__ uint64_t small_root_mul(uint64_t x)
+{
+    constexpr int L = gl::clog2(gl::comega(D + 1));
+    static_assert(L >= 0, "the 2^(D+1)-th root of unity of the transform is a power of two");
+    constexpr int E0 = (V * L) % 192;
+    constexpr int E = INV ? (192 - E0) % 192 : E0;
+    if constexpr (E == 0) return x;
+    else return gl::mul_pow2<E>(x);
+}
+
+template <int DD, bool INV>
+__device__ __forceinline__ void last_stages(uint64_t (&x)[kNttPerThread])
+{
+    static_for<0, kNttPerThread>([&](auto ec) {
+        constexpr int e = decltype(ec)::value;
+        if constexpr (!((e >> DD) & 1)) {
+            constexpr int f = e | (1 << DD);
+            const uint64_t y0 = x[e], y1 = x[f];
+            x[e] = gl::add(y0, y1);
+            x[f] = small_root_mul<DD, (e & ((1 << DD) - 1)), INV>(gl::sub(y0, y1));
+        }
+    });
+    if constexpr (DD > 0) last_stages<DD - 1, INV>(x);
+}
+
+// The round on stages K-1..0 of a contiguous pass (lane t holds the 2^K
+// elements u = (other << K) | rb, other = t | group << 8): ntt_round /
+// ntt_round4 with b0 = 0, twiddles folded in at compile time.
+template <int K, bool INV>
+__device__ __forceinline__ void ntt_round_last(uint64_t *lds, uint32_t t)
+{
+    constexpr int E = kNttPerThread;
+    uint64_t x[E];
+    uint32_t uu[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const uint32_t rb = (uint32_t)e & ((1u << K) - 1);
+        const uint32_t other = t | (((uint32_t)e >> K) << 8);
+        uu[e] = (other << K) | rb;
+        x[e] = lds[lds_slot(uu[e])];
+    }
+    last_stages<K - 1, INV>(x);
+#pragma unroll
+    for (int e = 0; e < E; e++) lds[lds_slot(uu[e])] = x[e];
+}
+
+template <int K>
+__device__ __forceinline__ void ntt_round_last(uint64_t *lds, uint32_t t, bool inverse)
+{
+    if (inverse) ntt_round_last<K, true>(lds, t);
+    else ntt_round_last<K, false>(lds, t);
+}
+
 // `passes` (npasses > 0, device memory) lists independent passes sorted by
 // tile_begin; with npasses == 0 the launch runs the single pass `one`.
 // CONTIG: every pass of the launch is contiguous (s_lo == 0), else strided.
@@ -214,12 +286,19 @@ k_ntt_tiles(const NttPass *__restrict__ passes, uint32_t npasses, NttPass one)
     uint32_t rem = a.m;
     while (rem >= 4) {
         rem -= 4;
-        ntt_round4<CONTIG>(lds, a, t, rem, jpos, lo_base);
+        if (CONTIG && rem == 0) ntt_round_last<4>(lds, t, a.inverse);
+        else ntt_round4<CONTIG>(lds, a, t, rem, jpos, lo_base);
         __syncthreads();
     }
-    if (rem == 3) ntt_round<CONTIG, 3>(lds, a, t, 0, jpos, lo_base);
-    else if (rem == 2) ntt_round<CONTIG, 2>(lds, a, t, 0, jpos, lo_base);
-    else if (rem == 1) ntt_round<CONTIG, 1>(lds, a, t, 0, jpos, lo_base);
+    if constexpr (CONTIG) {
+        if (rem == 3) ntt_round_last<3>(lds, t, a.inverse);
+        else if (rem == 2) ntt_round_last<2>(lds, t, a.inverse);
+        else if (rem == 1) ntt_round_last<1>(lds, t, a.inverse);
+    } else {
+        if (rem == 3) ntt_round<CONTIG, 3>(lds, a, t, 0, jpos, lo_base);
+        else if (rem == 2) ntt_round<CONTIG, 2>(lds, a, t, 0, jpos, lo_base);
+        else if (rem == 1) ntt_round<CONTIG, 1>(lds, a, t, 0, jpos, lo_base);
+    }
     if (rem) __syncthreads();
 
     // ---- store, coalesced
