@@ -73,6 +73,21 @@ __device__ __forceinline__ uint64_t tile_index(const NttPass &a, uint64_t tile, 
     return (tile << kNttTileLog) + u;
 }
 
+// Index of tile position u = t + 256 i as tile_index(a, tile, t) plus this
+// wave-uniform step: the lane id and i land in separate bit fields of the
+// position (contiguous: u itself; strided: q = u mod 2^c, j = u >> c).
+__device__ __forceinline__ uint64_t tile_step(const NttPass &a, int i)
+{
+    constexpr uint32_t lt = 8;  // log2 kNttThreads
+    static_assert((1 << lt) == kNttThreads, "256 lanes per tile");
+    if (a.s_lo == 0) return (uint64_t)i << lt;
+    if (a.c >= lt) {
+        const uint32_t cb = a.c - lt;
+        return ((uint64_t)(i & ((1 << cb) - 1)) << lt) + ((uint64_t)(i >> cb) << a.s_lo);
+    }
+    return (uint64_t)i << (lt - a.c + a.s_lo);
+}
+
 // One register round: stages on j-bits [b0, b0+K) (u-bits [RB, RB+K)).
 // Lane t holds the 2^K elements that differ in those bits, repeated over
 // 2^(4-K) groups taken from the extra element bits; the other u-bits come
@@ -256,22 +271,52 @@ k_ntt_tiles(const NttPass *__restrict__ passes, uint32_t npasses, NttPass one)
     const uint64_t tile = blockIdx.x - a.tile_begin;
     const uint32_t t = threadIdx.x;
 
-    // ---- load, coalesced: lane t takes positions t + 256 i, all 16 in flight
+    // ---- load, coalesced: lane t takes positions t + 256 i, all 16 in flight,
+    // at its own index plus a uniform step.  Only a contiguous pass over
+    // blocks shorter than a tile can end inside a tile (strided passes have
+    // L > 12, so every tile is full).
+    const uint64_t ibase = tile_index(a, tile, t);
+    const bool ragged = CONTIG && (a.total & ((1ull << kNttTileLog) - 1)) != 0;
+    const gptr<uint64_t> bufp = gmem(a.buf) + ibase;
+    // A strided pass reading through a mode is the first pass (s_lo + m = L):
+    // its tile lies in one block of 2^L, so the LDE source row and the factor
+    // (or coset) row follow the index at fixed offsets.
+    gptr<const uint64_t> srcp = nullptr, facp = nullptr;
+    if (!CONTIG && a.mode != kSrcInPlace) {
+        const uint64_t blk = ibase >> a.L;
+        const uint64_t pos0 = ibase - (blk << a.L);
+        if (a.mode == kSrcLde) {
+            // first `rate` DIF stages of the zero-padded column: block b of
+            // column col holds a_i * F[b*n + i], F the resident factor table
+            srcp = gmem(a.src) + ((blk >> a.rate) << a.L) + pos0;
+            facp = gmem(a.coset) + ((blk & ((1ull << a.rate) - 1)) << a.L) + pos0;
+        } else {
+            facp = gmem(a.coset) + pos0;
+        }
+    }
 #pragma unroll
     for (int i = 0; i < kNttPerThread; i++) {
         const uint32_t u = t + kNttThreads * i;
-        const uint64_t idx = tile_index(a, tile, u);
+        const uint64_t st = tile_step(a, i);
         uint64_t v = 0;
-        if (idx < a.total) {
+        if constexpr (CONTIG) {
+            const uint64_t idx = ibase + st;
+            if (!ragged || idx < a.total) {
+                if (a.mode == kSrcLde) {
+                    const uint64_t n = 1ull << a.L;
+                    const uint64_t col = idx >> (a.L + a.rate), pos = idx & (n - 1);
+                    v = gl::mul(gmem(a.src)[(col << a.L) + pos], gmem(a.coset)[idx & ((n << a.rate) - 1)]);
+                } else {
+                    v = bufp[st];
+                    if (a.mode == kSrcCoset) v = gl::mul(v, gmem(a.coset)[idx & ((1ull << a.L) - 1)]);
+                }
+            }
+        } else {
             if (a.mode == kSrcLde) {
-                // first `rate` DIF stages of the zero-padded column: block b of
-                // column col holds a_i * F[b*n + i], F the resident factor table
-                const uint64_t n = 1ull << a.L;
-                const uint64_t col = idx >> (a.L + a.rate), pos = idx & (n - 1);
-                v = gl::mul(gmem(a.src)[(col << a.L) + pos], gmem(a.coset)[idx & ((n << a.rate) - 1)]);
+                v = gl::mul(srcp[st], facp[st]);
             } else {
-                v = gmem(a.buf)[idx];
-                if (a.mode == kSrcCoset) v = gl::mul(v, gmem(a.coset)[idx & ((1ull << a.L) - 1)]);
+                v = bufp[st];
+                if (a.mode == kSrcCoset) v = gl::mul(v, facp[st]);
             }
         }
         lds[lds_slot(u)] = v;
@@ -302,11 +347,11 @@ k_ntt_tiles(const NttPass *__restrict__ passes, uint32_t npasses, NttPass one)
     if (rem) __syncthreads();
 
     // ---- store, coalesced
-#pragma unroll 4
+#pragma unroll
     for (int i = 0; i < kNttPerThread; i++) {
         const uint32_t u = t + kNttThreads * i;
-        const uint64_t idx = tile_index(a, tile, u);
-        if (idx < a.total) gmem(a.buf)[idx] = lds[lds_slot(u)];
+        const uint64_t st = tile_step(a, i);
+        if (!ragged || ibase + st < a.total) bufp[st] = lds[lds_slot(u)];
     }
 }
 
