@@ -12,6 +12,9 @@ tree) and is run once in the build container:
 
 LDE / transpose fixtures come from the oracle (the reference has no CPU LDE);
 the oracle's LDE is pinned by the brute-force DFT and closed-form tests.
+Whole-commit fixtures chain the two: the oracle's coset LDE and leaf rows,
+then the reference's Merkle over those rows (the headline 135 x 2^15 rate-3
+commit and one 400 x 2^20 rate-1 job, a few minutes of CPU).
 
 Inputs are not stored: they are regenerated from (count, seed) by the shared
 splitmix64 generator (oracle orc_fill_canonical), so the fixture file stays
@@ -45,6 +48,11 @@ LARGE = [  # hashed outputs: 2^13 x 135, cap height 4, every hash
 ]
 LDE = [  # (lg_n, rate, cols, coset, seed)
     (4, 1, 3, True, 31), (6, 2, 2, True, 32), (8, 3, 4, False, 33), (10, 1, 5, True, 34),
+]
+COMMITS = [  # whole commits (coset LDE with shift 7 -> leaf rows -> Merkle): (cols, lg_n, rate, cap, hash, seed)
+    (135, 15, 3, 4, 0, 2001),  # the headline commit: 135 x 2^15, rate 3, Poseidon
+    (135, 15, 3, 4, 3, 2002),  # the same shape, Poseidon2
+    (400, 20, 1, 4, 3, 2003),  # one job of the 8 x 400 x 2^20 batch (2^21 leaves of 400), Poseidon2
 ]
 
 
@@ -90,6 +98,17 @@ def main():
         out["lde"].append({"lg_n": lg_n, "rate": rate, "cols": cols, "coset": coset, "seed": seed,
                            "lde_sha256": sha(lde), "leaves_sha256": sha(leaves),
                            "lde_head": [str(int(v)) for v in lde[:8]]})
+    out["commits"] = []
+    for cols, lg_n, rate, h, ht, seed in COMMITS:
+        print("commit", cols, lg_n, rate, h, ht, flush=True)
+        x = ob.canonical(cols << lg_n, seed)
+        leaves = ob.transpose_rev(ob.lde(x, lg_n, rate, cols), lg_n + rate, cols)
+        del x
+        d, c = ref_merkle(ref, leaves, cols, h, ht)
+        out["commits"].append({"cols": cols, "lg_n": lg_n, "rate": rate, "cap_height": h, "hash": ht, "seed": seed,
+                               "leaves_sha256": sha(leaves), "digests_sha256": sha(d),
+                               "caps": [str(int(v)) for v in c]})
+        del leaves, d
     with open(os.path.join(HERE, "fixtures.json"), "w") as f:
         json.dump(out, f, indent=1)
     print("wrote", os.path.join(HERE, "fixtures.json"))

@@ -76,3 +76,62 @@ def test_gpu_lde_fixture(zk, rec):
     leaves = np.zeros_like(out)
     zk.transpose_rev_batch(0, leaves, out, lg_n + rate, zk.TransposeConfig(batches=cols))
     assert sha(leaves) == rec["leaves_sha256"]
+
+
+# ---------------------------------------------------------------- whole commits
+
+def commit_id(r):
+    return f"{r['cols']}x2e{r['lg_n']}r{r['rate']}H{r['hash']}"
+
+
+def check_commit(rec, leaves, dig, cap):
+    assert np.array_equal(cap, u64s(rec["caps"]))
+    assert sha(dig) == rec["digests_sha256"]
+    if leaves is not None:
+        assert sha(leaves) == rec["leaves_sha256"]
+
+
+SMALL_COMMITS = [r for r in FIX["commits"] if r["cols"] << (r["lg_n"] + r["rate"]) <= 1 << 26]
+
+
+@pytest.mark.parametrize("rec", SMALL_COMMITS, ids=commit_id)
+def test_oracle_whole_commit_matches_reference_fixture(rec):
+    """The oracle's coset LDE and leaf rows, then its Merkle, against the
+    fixture whose Merkle the reference's merkle.cpp computed."""
+    cols, lg_n, rate = rec["cols"], rec["lg_n"], rec["rate"]
+    leaves = ob.transpose_rev(ob.lde(ob.canonical(cols << lg_n, rec["seed"]), lg_n, rate, cols), lg_n + rate, cols)
+    dig, cap = ob.merkle(leaves, cols, rec["cap_height"], rec["hash"])
+    check_commit(rec, leaves, dig, cap)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rec", FIX["commits"], ids=commit_id)
+def test_gpu_whole_commit_matches_reference_fixture(zk, rec):
+    """The batched engine at full size (the headline commit and one job of
+    the 400 x 2^20 batch), every leaf row, digest and cap."""
+    zk.init_coset_rs(0, 24, 7)
+    cols, lg_n, rate, h, ht = rec["cols"], rec["lg_n"], rec["rate"], rec["cap_height"], rec["hash"]
+    print(f"whole commit {commit_id(rec)}", flush=True)
+    c = zk.Commit(ob.canonical(cols << lg_n, rec["seed"]), cols, lg_n, rate, h, ht)
+    zk.commit_batch([c])
+    check_commit(rec, *c.results())
+
+
+@pytest.mark.gpu
+def test_gpu_plain_headline_commit_matches_reference_fixture(zk):
+    """The same headline commit through the plain C-ABI sequence:
+    compute_batched_lde -> compute_transpose_rev -> fill_digests (device)."""
+    rec = FIX["commits"][0]
+    zk.init_coset_rs(0, 24, 7)
+    cols, lg_n, rate, h, ht = rec["cols"], rec["lg_n"], rec["rate"], rec["cap_height"], rec["hash"]
+    lg, N = lg_n + rate, 1 << (lg_n + rate)
+    d_in = zk.DeviceBuffer.from_host(ob.canonical(cols << lg_n, rec["seed"]))
+    d_lde, d_leaves = zk.DeviceBuffer(cols * N), zk.DeviceBuffer(cols * N)
+    zk.lde_batch(0, d_lde, d_in, lg_n, zk.NTTConfig(batches=cols, extension_rate_bits=rate, with_coset=True,
+                                                   are_inputs_on_device=True, are_outputs_on_device=True))
+    zk.transpose_rev_batch(0, d_leaves, d_lde, lg, zk.TransposeConfig(batches=cols, are_inputs_on_device=True,
+                                                                      are_outputs_on_device=True))
+    nc, nd = 1 << h, 2 * (N - (1 << h))
+    d_dig, d_cap = zk.DeviceBuffer(nd * 4), zk.DeviceBuffer(nc * 4)
+    zk.fill_digests_buf_linear_gpu_with_gpu_ptr(d_dig, d_cap, d_leaves, nd, nc, N, cols, h, ht, 0)
+    check_commit(rec, d_leaves.to_host(), d_dig.to_host(nd * 4), d_cap.to_host())
