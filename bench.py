@@ -72,6 +72,8 @@ def parse():
     ap.add_argument("--schedule", default=None, metavar="S,E,T",
                     help="batched 135 x 2^15 schedule (ntt_strided_max,ntt_even,merkle_tile_log) instead of autotuning")
     ap.add_argument("--no-extra", action="store_true", help="skip the 64-job and 8 x 400 x 2^20 batches")
+    ap.add_argument("--legs", default="batch64,large,hashers,ntt",
+                    help="extra legs to run: batch64, large, hashers, ntt (comma-separated)")
     ap.add_argument("--extra-steps", type=int, default=3, help="timed steps of those batches")
     ap.add_argument("--cpu-runs", type=int, default=5, help="whole CPU commits timed (median reported)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -188,6 +190,57 @@ def plain_vs_batched(zk, np, shapes, steps, warmup, seed, tune):
             "batched_over_plain": round(plain_s / batch_s, 3), "batched_equals_plain": bool(same),
             "ms_per_batch": round(batch_s * 1e3, 2), "autotune_s": round(tune_s, 2),
             "schedules (ntt_strided_max, ntt_even, merkle_tile_log)": [list(t) for t in schedules]}
+
+
+def merkle_hashers(zk, np, steps, lg=20, cols=135, h=4):
+    """The Merkle forest alone through fill_digests_buf_linear_gpu_with_gpu_ptr
+    (device pointers, row-major leaves resident) for every hash type: 2^lg
+    leaves of `cols` elements, cap height h.  Per hash: ms per tree and
+    leaves/s; the Goldilocks sponges (Poseidon, Poseidon2, Monolith; rate 8)
+    also permutations/s (ceil(cols/8) per leaf + one per internal node)."""
+    L = 1 << lg
+    leaves = zk.DeviceBuffer(cols * L)
+    leaves.fill_random(77)
+    nc, nd = 1 << h, 2 * (L - (1 << h))
+    dig, cap = zk.DeviceBuffer(nd * 4), zk.DeviceBuffer(nc * 4)
+    perms = L * ((cols + 7) // 8) + (L - nc)
+    names = {zk.HASH_POSEIDON: "poseidon", zk.HASH_KECCAK: "keccak256", zk.HASH_POSEIDON_BN128: "poseidon_bn128",
+             zk.HASH_POSEIDON2: "poseidon2", zk.HASH_MONOLITH: "monolith"}
+    out = {"workload": f"Merkle forest only: 2^{lg} leaves x {cols}, cap_height {h}, device pointers"}
+    for ht, name in names.items():
+        def tree():
+            zk.fill_digests_buf_linear_gpu_with_gpu_ptr(dig, cap, leaves, nd, nc, L, cols, h, ht, 0)
+        t = _timed(zk, tree, steps, 1)
+        rec = {"ms_per_tree": round(t * 1e3, 3), "leaves_per_s": round(L / t)}
+        if ht in (zk.HASH_POSEIDON, zk.HASH_POSEIDON2, zk.HASH_MONOLITH):
+            rec["perms_per_s"] = round(perms / t)
+        out[name] = rec
+    return out
+
+
+def ntt_legs(zk, np, steps, lg=20, cols=64):
+    """compute_batched_ntt forward and inverse (natural order in and out) of
+    `cols` columns of 2^lg on device buffers, and a salted coset LDE (rate 1,
+    4 salt columns) of the same columns at half the length: ms per call and
+    the algorithmic rate (each column read and written once)."""
+    n = cols << lg
+    buf = zk.DeviceBuffer(n)
+    buf.fill_random(91)
+    dev = dict(are_inputs_on_device=True, are_outputs_on_device=True)
+    fwd = _timed(zk, lambda: zk.ntt_batch(0, buf, lg, zk.NTTConfig(batches=cols, **dev)), steps, 1)
+    inv = _timed(zk, lambda: zk.intt_batch(0, buf, lg, zk.NTTConfig(batches=cols, **dev)), steps, 1)
+    half = zk.DeviceBuffer(cols << (lg - 1))
+    half.fill_random(92)
+    salt = 4
+    lde_out = zk.DeviceBuffer((cols + salt) << lg)
+    lde = _timed(zk, lambda: zk.lde_batch(0, lde_out, half, lg - 1, zk.NTTConfig(
+        batches=cols, extension_rate_bits=1, with_coset=True, salt_size=salt, **dev)), steps, 1)
+    gbs = lambda nbytes, t: round(nbytes / t / 1e9, 1)
+    return {"workload": f"{cols} columns x 2^{lg}: NTT forward and inverse (natural order); coset LDE "
+                        f"2^{lg - 1} -> 2^{lg}, rate 1, {salt} salt columns",
+            "ntt_forward_ms": round(fwd * 1e3, 3), "ntt_forward_gbs": gbs(16 * n, fwd),
+            "ntt_inverse_ms": round(inv * 1e3, 3), "ntt_inverse_gbs": gbs(16 * n, inv),
+            "lde_salted_ms": round(lde * 1e3, 3), "lde_salted_gbs": gbs(8 * (n // 2) + 8 * ((cols + salt) << lg), lde)}
 
 
 def batch64_mixed(zk, np, steps, warmup, tune, seed=3):
@@ -440,8 +493,15 @@ def main():
     # ---- heterogeneous 64-job batch and the large Poseidon2 batch, both ways
     extra = {}
     if not args.no_extra:
-        extra["batch64"] = batch64_mixed(zk, np, args.extra_steps, 1, not args.no_tune)
-        extra["large_poseidon2"] = large_poseidon2_batch(zk, np, args.extra_steps, 1, not args.no_tune)
+        legs = set(args.legs.split(","))
+        if "batch64" in legs:
+            extra["batch64"] = batch64_mixed(zk, np, args.extra_steps, 1, not args.no_tune)
+        if "large" in legs:
+            extra["large_poseidon2"] = large_poseidon2_batch(zk, np, args.extra_steps, 1, not args.no_tune)
+        if "hashers" in legs:
+            extra["merkle_hashers"] = merkle_hashers(zk, np, args.extra_steps)
+        if "ntt" in legs:
+            extra["ntt"] = ntt_legs(zk, np, args.extra_steps)
 
     # ---- CPU baseline: whole commits of the same input on the host cores
     cpu = None
