@@ -54,22 +54,33 @@ __device__ __forceinline__ void poseidon_mds(uint64_t s[12], const uint64_t *k)
         lo[i] = gl::lo32(s[i]);
         hi[i] = gl::hi32(s[i]);
     }
+    // two rows at a time: four accumulation chains interleaved, so no
+    // multiply-add reads an accumulator written by the instruction before
+    // last (which would cost a wait state)
 #pragma unroll
-    for (int r = 0; r < 12; r++) {
-        const uint64_t kr = k[r];
-        uint64_t A = gl::lo32(kr), B = gl::hi32(kr);
+    for (int r = 0; r < 12; r += 2) {
+        uint64_t A[2], B[2];
 #pragma unroll
-        for (int i = 0; i < 12; i++) {
-            const int j = (i + r) % 12;
-            const uint32_t c0 = (uint32_t)ZK_POSEIDON_MDS_CIRC[i] + (i == 0 ? (uint32_t)ZK_POSEIDON_MDS_DIAG[r] : 0u);
-            // the first product takes the SGPR constant pair as its addend, so
-            // its coefficient stays an inline constant (one SGPR per VALU op)
-            const uint32_t c = i == 0 ? c0 : gl::opaque(c0);
-            // each step pinned: one v_mad_u64_u32 per product, no regrouping
-            A = gl::vopaque64((uint64_t)lo[j] * c + A);
-            B = gl::vopaque64((uint64_t)hi[j] * c + B);
+        for (int q = 0; q < 2; q++) {
+            A[q] = gl::lo32(k[r + q]);
+            B[q] = gl::hi32(k[r + q]);
         }
-        s[r] = limb_combine(A, B);
+#pragma unroll
+        for (int i = 0; i < 12; i++)
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const int row = r + q, j = (i + row) % 12;
+                const uint32_t c0 =
+                    (uint32_t)ZK_POSEIDON_MDS_CIRC[i] + (i == 0 ? (uint32_t)ZK_POSEIDON_MDS_DIAG[row] : 0u);
+                // the first product takes the SGPR constant pair as its addend,
+                // so its coefficient stays an inline constant (one SGPR per VALU op)
+                const uint32_t c = i == 0 ? c0 : gl::opaque(c0);
+                // each step pinned: one v_mad_u64_u32 per product, no regrouping
+                A[q] = gl::vopaque64((uint64_t)lo[j] * c + A[q]);
+                B[q] = gl::vopaque64((uint64_t)hi[j] * c + B[q]);
+            }
+#pragma unroll
+        for (int q = 0; q < 2; q++) s[r + q] = limb_combine(A[q], B[q]);
     }
 }
 
