@@ -185,6 +185,49 @@ __device__ __forceinline__ uint64_t reduce128_lazy(uint64_t lo, uint64_t hi)
     return pack(lo32(t), rhi);
 }
 
+// Two independent reduce128_lazy at once: the same instructions with the two
+// reductions interleaved, so every lane-mask read already sits two
+// instructions after its write (one s_nop 0 per pair of steps instead of an
+// s_nop 1 per step).
+__device__ __forceinline__ void reduce128_lazy2(uint64_t lo0, uint64_t hi0, uint64_t lo1, uint64_t hi1, uint64_t &r0,
+                                                uint64_t &r1)
+{
+    uint64_t u0, u1, sc0, sc1;
+    uint32_t cv0, cv1;
+    asm("v_mad_u64_u32 %[u0], %[sc0], %[a0], -1, %[l0]\n\t"
+        "v_mad_u64_u32 %[u1], %[sc1], %[a1], -1, %[l1]\n\t"
+        "s_nop 0\n\t"
+        "v_cndmask_b32_e64 %[cv0], 0, 1, %[sc0]\n\t"
+        "v_cndmask_b32_e64 %[cv1], 0, 1, %[sc1]"
+        : [u0] "=&v"(u0), [u1] "=&v"(u1), [sc0] "=&s"(sc0), [sc1] "=&s"(sc1), [cv0] "=&v"(cv0), [cv1] "=&v"(cv1)
+        : [a0] "v"(lo32(hi0)), [l0] "v"(lo0), [a1] "v"(lo32(hi1)), [l1] "v"(lo1));
+    uint32_t zl0, zh0, cb0, zl1, zh1, cb1;
+    uint64_t b10, b0, j0, b11, b1, j1;
+    asm("v_sub_co_u32_e64 %[zl0], %[b10], %[ul0], %[h10]\n\t"
+        "v_sub_co_u32_e64 %[zl1], %[b11], %[ul1], %[h11]\n\t"
+        "s_nop 0\n\t"
+        "v_subb_co_u32_e64 %[zh0], %[b0], %[uh0], 0, %[b10]\n\t"
+        "v_subb_co_u32_e64 %[zh1], %[b1], %[uh1], 0, %[b11]\n\t"
+        "s_nop 0\n\t"
+        "v_subb_co_u32_e64 %[cb0], %[j0], %[cv0], 0, %[b0]\n\t"
+        "v_subb_co_u32_e64 %[cb1], %[j1], %[cv1], 0, %[b1]"
+        : [zl0] "=&v"(zl0), [zh0] "=&v"(zh0), [cb0] "=&v"(cb0), [zl1] "=&v"(zl1), [zh1] "=&v"(zh1),
+          [cb1] "=&v"(cb1), [b10] "=&s"(b10), [b0] "=&s"(b0), [j0] "=&s"(j0), [b11] "=&s"(b11), [b1] "=&s"(b1),
+          [j1] "=&s"(j1)
+        : [ul0] "v"(lo32(u0)), [uh0] "v"(hi32(u0)), [h10] "v"(hi32(hi0)), [cv0] "v"(cv0), [ul1] "v"(lo32(u1)),
+          [uh1] "v"(hi32(u1)), [h11] "v"(hi32(hi1)), [cv1] "v"(cv1));
+    uint64_t t0, t1, tj0, tj1;
+    asm("v_mad_i64_i32 %[t0], %[tj0], %[cb0], -1, %[z0]\n\t"
+        "v_mad_i64_i32 %[t1], %[tj1], %[cb1], -1, %[z1]"
+        : [t0] "=&v"(t0), [t1] "=&v"(t1), [tj0] "=&s"(tj0), [tj1] "=&s"(tj1)
+        : [cb0] "v"(cb0), [z0] "v"(pack(zl0, zh0)), [cb1] "v"(cb1), [z1] "v"(pack(zl1, zh1)));
+    uint32_t rh0, rh1;  // high-word adds kept in asm (see reduce128_lazy)
+    asm("v_add_u32 %[r], %[a], %[b]" : [r] "=v"(rh0) : [a] "v"(hi32(t0)), [b] "v"(cb0));
+    asm("v_add_u32 %[r], %[a], %[b]" : [r] "=v"(rh1) : [a] "v"(hi32(t1)), [b] "v"(cb1));
+    r0 = pack(lo32(t0), rh0);
+    r1 = pack(lo32(t1), rh1);
+}
+
 // x mod p for any x < 2^64 (x >= p iff x + EPS carries).
 __device__ __forceinline__ uint64_t canon(uint64_t x)
 {
@@ -200,6 +243,17 @@ __device__ __forceinline__ uint64_t mul_lazy(uint64_t a, uint64_t b)
     uint64_t lo, hi;
     mul128(a, b, lo, hi);
     return reduce128_lazy(lo, hi);
+}
+
+// Two independent lazy products (their multiply-add chains and reductions
+// interleave).
+__device__ __forceinline__ void mul_lazy2(uint64_t a0, uint64_t b0, uint64_t a1, uint64_t b1, uint64_t &r0,
+                                          uint64_t &r1)
+{
+    uint64_t lo0, hi0, lo1, hi1;
+    mul128(a0, b0, lo0, hi0);
+    mul128(a1, b1, lo1, hi1);
+    reduce128_lazy2(lo0, hi0, lo1, hi1, r0, r1);
 }
 
 // a * b, canonical (the lazy product's 7-instruction reduction plus the
@@ -261,13 +315,24 @@ __device__ __forceinline__ uint64_t mul_pow2(uint64_t x)
 }
 
 // x^7 with the addition chain x2 = x^2, x3 = x*x2, x4 = x2^2, x7 = x3*x4;
-// lazy in and out (any x < 2^64; result below 2^64, congruent mod p).
+// lazy in and out (any x < 2^64; result below 2^64, congruent mod p).  x3
+// and x4 are independent products, computed as a pair.
 __device__ __forceinline__ uint64_t pow7_lazy(uint64_t x)
 {
     const uint64_t x2 = mul_lazy(x, x);
-    const uint64_t x3 = mul_lazy(x2, x);
-    const uint64_t x4 = mul_lazy(x2, x2);
+    uint64_t x3, x4;
+    mul_lazy2(x2, x, x2, x2, x3, x4);
     return mul_lazy(x3, x4);
+}
+
+// x^7 and y^7 in lockstep (every product paired).
+__device__ __forceinline__ void pow7_lazy2(uint64_t &x, uint64_t &y)
+{
+    uint64_t x2, y2, x3, y3, x4, y4;
+    mul_lazy2(x, x, y, y, x2, y2);
+    mul_lazy2(x2, x, y2, y, x3, y3);
+    mul_lazy2(x2, x2, y2, y2, x4, y4);
+    mul_lazy2(x3, x4, y3, y4, x, y);
 }
 
 // x^7, canonical.

@@ -97,7 +97,7 @@ __device__ __forceinline__ void poseidon_permute(uint64_t s[12])
 #pragma unroll 1
     for (int r = 0; r < 4; r++) {
 #pragma unroll
-        for (int i = 0; i < 12; i++) s[i] = gl::pow7_lazy(s[i]);
+        for (int i = 0; i < 12; i += 2) gl::pow7_lazy2(s[i], s[i + 1]);
         poseidon_mds(s, c_poseidon_rc + 12 * (r + 1));
     }
 #pragma unroll 1
@@ -111,7 +111,7 @@ __device__ __forceinline__ void poseidon_permute(uint64_t s[12])
 #pragma unroll 1
     for (int r = 26; r < 30; r++) {
 #pragma unroll
-        for (int i = 0; i < 12; i++) s[i] = gl::pow7_lazy(s[i]);
+        for (int i = 0; i < 12; i += 2) gl::pow7_lazy2(s[i], s[i + 1]);
         poseidon_mds(s, c_poseidon_rc + 12 * (r + 1));
     }
 }
@@ -197,19 +197,30 @@ __device__ __forceinline__ void p2_internal(uint64_t s[12])
 __device__ __forceinline__ void poseidon2_permute(uint64_t s[12])
 {
     p2_external(s);
-    // rounds 0-3 and 26-29 external (all 12 S-boxes), 4-25 internal (s0 only)
-#pragma unroll 1
-    for (int r = 0; r < 30; r++) {
+    // rounds 0-3 and 26-29 external (all 12 S-boxes, in pairs), 4-25
+    // internal (s0 only)
+    auto external_round = [&](int r) {
         const uint64_t *rc = c_poseidon2_rc + 12 * r;
-        s[0] = gl::pow7_lazy(gl::add(s[0], rc[0]));
-        if (r < 4 || r >= 26) {
 #pragma unroll
-            for (int i = 1; i < 12; i++) s[i] = gl::pow7_lazy(gl::add(s[i], rc[i]));
-            p2_external(s);
-        } else {
+        for (int i = 0; i < 12; i += 2) {
+            s[i] = gl::add(s[i], rc[i]);
+            s[i + 1] = gl::add(s[i + 1], rc[i + 1]);
+            gl::pow7_lazy2(s[i], s[i + 1]);
+        }
+        p2_external(s);
+    };
+#pragma unroll 1
+    for (int r = 0; r < 4; r++) external_round(r);
+#pragma unroll 1
+    for (int r = 4; r < 26; r += 2) {  // two per iteration: no loop-carried copies
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            s[0] = gl::pow7_lazy(gl::add(s[0], c_poseidon2_rc[12 * (r + h)]));
             p2_internal(s);
         }
     }
+#pragma unroll 1
+    for (int r = 26; r < 30; r++) external_round(r);
 }
 
 template <int HT>
