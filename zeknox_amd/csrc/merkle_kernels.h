@@ -39,10 +39,11 @@
 
 namespace zkx {
 
-// Occupancy hint per hash type: the Poseidon and Poseidon2 kernels fit six
-// waves per SIMD in 80 VGPRs (measured 3-4% faster on the 135 x 2^18 leaf
-// launch than the compiler's own 96 VGPRs at five waves); the others would
-// spill, and keep the compiler's choice.
+// Occupancy hint per hash type: six waves per SIMD (80 VGPRs) for Poseidon
+// (measured 3-4% faster on the 135 x 2^18 leaf launch than the compiler's
+// own 96 VGPRs at five waves) and Poseidon2 (8 VGPRs spilled outside the round
+// loop, and as fast as five waves without the spill); the others would spill
+// more, and keep the compiler's choice.
 constexpr int merkle_waves_per_eu(int ht) { return (ht == kPoseidon || ht == kPoseidon2) ? 6 : 1; }
 
 // Digest slot of node `idx` of the level of width 2^lg_w in subtree `sub`.
