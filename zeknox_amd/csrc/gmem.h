@@ -21,4 +21,18 @@ __device__ __forceinline__ gptr<T> gmem(T *p)
     return (gptr<T>)p;
 }
 
+// Read-only data that no kernel writes while it runs (descriptor lists the
+// host fills before the launch), viewed through the constant address space:
+// wave-uniform reads become scalar loads into SGPRs even after the kernel
+// has stored to global memory, which would otherwise keep the compiler on
+// vector loads.
+template <class T>
+using cptr = __attribute__((address_space(4))) const T *;
+
+template <class T>
+__device__ __forceinline__ cptr<T> cmem(const T *p)
+{
+    return (cptr<T>)p;
+}
+
 }  // namespace zkx

@@ -4,6 +4,8 @@
 #include "ntt.h"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -13,12 +15,40 @@ namespace zkx {
 
 namespace {
 
+// Work-groups of k_ntt_contig_pipe resident at once on the current device
+// (its persistent grid).
+uint64_t pipe_groups()
+{
+    static std::mutex mu;
+    static std::map<int, uint64_t> cache;
+    int dev = 0;
+    ZKX_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find(dev);
+    if (it != cache.end()) return it->second;
+    int per_cu = 0, cus = 0;
+    ZKX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ntt_contig_pipe, kNttThreads, 0));
+    ZKX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const uint64_t n = (uint64_t)std::max(per_cu, 1) * (uint64_t)std::max(cus, 1);
+    cache[dev] = n;
+    return n;
+}
+
+void launch_contig(hipStream_t stream, const NttPass *dev_list, uint32_t n, const NttPass &one, uint64_t tiles)
+{
+    const uint64_t groups = std::min(tiles, pipe_groups());
+    hipLaunchKernelGGL(k_ntt_contig_pipe, dim3((uint32_t)groups), dim3(kNttThreads), 0, stream, dev_list, n, one,
+                       tiles);
+}
+
 // Launch one pass (a single descriptor passed by value).
 void launch_pass(Device &d, const NttPass &p, uint64_t tiles, const char *name)
 {
     if (tiles > 0x7FFFFFFFull) invalid("transform too large for one launch");
     ProfScope ps(name, d.stream);
-    if (p.s_lo == 0)
+    if (ntt_pipe_ok(&p, 1))
+        launch_contig(d.stream, nullptr, 0u, p, tiles);
+    else if (p.s_lo == 0)
         hipLaunchKernelGGL(k_ntt_tiles<true>, dim3((uint32_t)tiles), dim3(kNttThreads), 0, d.stream,
                            (const NttPass *)nullptr, 0u, p);
     else
@@ -42,13 +72,24 @@ void run_dif(Device &d, uint64_t *buf, uint64_t nblk, uint32_t L, uint32_t lg_tw
 
 }  // namespace
 
+bool ntt_pipe_ok(const NttPass *host_list, size_t n)
+{
+    for (size_t i = 0; i < n; i++) {
+        const NttPass &p = host_list[i];
+        if (p.s_lo != 0 || p.mode != kSrcInPlace || (p.total & ((1ull << kNttTileLog) - 1)) != 0) return false;
+    }
+    return n > 0;
+}
+
 void launch_ntt_list(hipStream_t stream, const NttPass *dev_list, uint32_t n, uint64_t blocks, bool contiguous,
-                     const char *name)
+                     bool pipe, const char *name)
 {
     if (blocks == 0) return;
     if (blocks > 0x7FFFFFFFull) invalid("transform batch too large for one launch");
     ProfScope ps(name, stream);
-    if (contiguous)
+    if (pipe)
+        launch_contig(stream, dev_list, n, NttPass{}, blocks);
+    else if (contiguous)
         hipLaunchKernelGGL(k_ntt_tiles<true>, dim3((uint32_t)blocks), dim3(kNttThreads), 0, stream, dev_list, n,
                            NttPass{});
     else

@@ -42,6 +42,15 @@ __device__ __forceinline__ uint64_t tw12_at(const NttPass &a, uint32_t k)
     return gmem(a.tw12)[a.inverse ? ((0u - k) & ((1u << kNttTileLog) - 1)) : k];
 }
 
+// The same twiddle, k < 2048, from the work-group's LDS copy of the pass's
+// direction (TWL) or from the global table.
+template <bool TWL>
+__device__ __forceinline__ uint64_t tw12_get(const NttPass &a, const uint64_t *twl, uint32_t k)
+{
+    if constexpr (TWL) return twl[k];
+    else return tw12_at(a, k);
+}
+
 // Twiddle table of a strided pass's stage d (butterfly span 2^(s_lo+d+1)):
 // w_{2^(s_lo+d+1)}^k sits at index k, so the 2^c consecutive columns of a
 // tile read consecutive words (rather than words 2^(lg_tw-s_lo-d-1) apart
@@ -93,9 +102,9 @@ __device__ __forceinline__ uint64_t tile_step(const NttPass &a, int i)
 // 2^(4-K) groups taken from the extra element bits; the other u-bits come
 // from the lane id.  The twiddle of stage d for element e depends only on
 // (j mod 2^d) << s_lo | lo.
-template <bool CONTIG, int K>
+template <bool CONTIG, int K, bool TWL = false>
 __device__ __forceinline__ void ntt_round(uint64_t *lds, const NttPass &a, uint32_t t, uint32_t b0, uint32_t jpos,
-                                          uint64_t lo_base)
+                                          uint64_t lo_base, const uint64_t *twl = nullptr)
 {
     constexpr int E = kNttPerThread;
     const uint32_t RB = jpos + b0;
@@ -122,7 +131,7 @@ __device__ __forceinline__ void ntt_round(uint64_t *lds, const NttPass &a, uint3
             const uint32_t j = !CONTIG ? (u >> a.c) : (u & ((1u << a.m) - 1));
             uint64_t w;
             if constexpr (CONTIG) {
-                w = tw12_at(a, (j & ((1u << d) - 1)) << (kNttTileLog - 1 - d));
+                w = tw12_get<TWL>(a, twl, (j & ((1u << d) - 1)) << (kNttTileLog - 1 - d));
             } else {
                 const uint64_t lo_part = lo_base + (u & ((1u << a.c) - 1));
                 const uint64_t ex = (((uint64_t)(j & ((1u << d) - 1))) << a.s_lo) | lo_part;
@@ -139,9 +148,9 @@ __device__ __forceinline__ void ntt_round(uint64_t *lds, const NttPass &a, uint3
 
 // Full-width round (K = 4, no extra element bits): the lane's other u-bits
 // are its id, so the 15 distinct twiddles of the round are loaded once.
-template <bool CONTIG>
+template <bool CONTIG, bool TWL = false>
 __device__ __forceinline__ void ntt_round4(uint64_t *lds, const NttPass &a, uint32_t t, uint32_t b0, uint32_t jpos,
-                                           uint64_t lo_base)
+                                           uint64_t lo_base, const uint64_t *twl = nullptr)
 {
     constexpr int E = kNttPerThread;
     const uint32_t RB = jpos + b0;
@@ -157,11 +166,12 @@ __device__ __forceinline__ void ntt_round4(uint64_t *lds, const NttPass &a, uint
     for (int dd = 3; dd >= 0; dd--) {
         const uint32_t d = b0 + (uint32_t)dd;
         const uint32_t lg = a.s_lo + d + 1;
-        uint64_t w[1 << dd];
+        uint64_t w[8];  // 2^dd used
         if constexpr (CONTIG) {
             // contiguous pass: w_{2^(d+1)}^x = w_4096^(x << (11 - d))
 #pragma unroll
-            for (int v = 0; v < (1 << dd); v++) w[v] = tw12_at(a, (j0 | ((uint32_t)v << b0)) << (kNttTileLog - 1 - d));
+            for (int v = 0; v < (1 << dd); v++)
+                w[v] = tw12_get<TWL>(a, twl, (j0 | ((uint32_t)v << b0)) << (kNttTileLog - 1 - d));
         } else {
             const uint64_t *tab = stage_table(a, lg);
 #pragma unroll
@@ -352,6 +362,149 @@ k_ntt_tiles(const NttPass *__restrict__ passes, uint32_t npasses, NttPass one)
         const uint32_t u = t + kNttThreads * i;
         const uint64_t st = tile_step(a, i);
         if (!ragged || ibase + st < a.total) bufp[st] = lds[lds_slot(u)];
+    }
+}
+
+// ---- contiguous passes, persistent and pipelined
+//
+// A contiguous pass is load -> up to 12 stages in LDS -> store per tile; a
+// work-group that runs one tile and exits leaves HBM idle while it computes.
+// Here each work-group walks tiles blockIdx.x, +gridDim.x, ... and issues the
+// global loads of its next tile before the rounds of the current one, so the
+// loads (16 per lane, in registers) are in flight while it computes.  The
+// stage twiddles come from an LDS copy of the w_4096 table, loaded once per
+// work-group, so the rounds issue no global load: vector-memory counters
+// retire in order, and a wait for a load issued in the rounds would also wait
+// for the prefetch.  Per tile:
+//   rounds(k) | per position: read result(k), write input(k+1), store result(k)
+//   | prefetch(k+2) | barrier
+// The kernel takes in-place passes over whole tiles only (ntt_pipe_ok):
+// input transforms and ragged ends would put branches between the loads and
+// stores, and across a branch the compiler loses its count of the memory
+// operations in flight and waits for all of them.
+
+constexpr int kNttPipeWaves = 3;  // per SIMD: 49 KB of LDS and 168 VGPRs per work-group
+constexpr int kTw12Half = 1 << (kNttTileLog - 1);
+
+__device__ __forceinline__ NttPass pass_at(const NttPass *passes, uint32_t npasses, const NttPass &one, uint64_t b)
+{
+    if (!npasses) return one;
+    const cptr<NttPass> cp = cmem(passes);
+    uint32_t lo = 0, hi = npasses;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (cp[mid].tile_begin <= b) lo = mid;
+        else hi = mid;
+    }
+    static_assert(sizeof(NttPass) % 8 == 0, "descriptor copied as 64-bit words");
+    constexpr int W = sizeof(NttPass) / 8;
+    const cptr<uint64_t> src = (cptr<uint64_t>)(cp + lo);
+    uint64_t w[W];
+#pragma unroll
+    for (int i = 0; i < W; i++) w[i] = src[i];
+    NttPass r;
+    __builtin_memcpy(&r, w, sizeof r);
+    return r;
+}
+
+// The LDS copy of the contiguous stage twiddles w_4096^k, k < 2048, in the
+// pass's direction: all eight loads per lane in flight, then the stores.
+__device__ __forceinline__ void load_twl(const NttPass &a, uint32_t t, uint64_t *twl)
+{
+    constexpr int R = kTw12Half / kNttThreads;
+    uint64_t v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) v[r] = tw12_at(a, t + kNttThreads * r);
+#pragma unroll
+    for (int r = 0; r < R; r++) twl[t + kNttThreads * r] = v[r];
+}
+
+// Words t + 256 i of the tile.
+__device__ __forceinline__ void contig_fetch(const NttPass &a, uint64_t tile, uint32_t t,
+                                             uint64_t (&raw)[kNttPerThread])
+{
+    const gptr<const uint64_t> src = gmem((const uint64_t *)a.buf) + (tile << kNttTileLog) + t;
+#pragma unroll
+    for (int i = 0; i < kNttPerThread; i++) raw[i] = src[i << 8];
+}
+
+__device__ __forceinline__ void contig_rounds(uint64_t *lds, const NttPass &a, uint32_t t, const uint64_t *twl)
+{
+    uint32_t rem = a.m;
+    while (rem >= 4) {
+        rem -= 4;
+        if (rem == 0) ntt_round_last<4>(lds, t, a.inverse);
+        else ntt_round4<true, true>(lds, a, t, rem, 0, 0, twl);
+        __syncthreads();
+    }
+    if (rem == 3) ntt_round_last<3>(lds, t, a.inverse);
+    else if (rem == 2) ntt_round_last<2>(lds, t, a.inverse);
+    else if (rem == 1) ntt_round_last<1>(lds, t, a.inverse);
+    if (rem) __syncthreads();
+}
+
+// Launch with at most `ntiles` work-groups over passes that satisfy
+// ntt_pipe_ok.
+__global__ void __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_eu(kNttPipeWaves)))
+k_ntt_contig_pipe(const NttPass *__restrict__ passes, uint32_t npasses, NttPass one, uint64_t ntiles)
+{
+    __shared__ uint64_t lds[(1 << kNttTileLog) + (1 << kNttTileLog) / 32];
+    __shared__ uint64_t twl[kTw12Half];
+    const uint32_t t = threadIdx.x;
+    const uint64_t b = blockIdx.x;
+    if (b >= ntiles) return;
+    NttPass a = pass_at(passes, npasses, one, b);
+    uint64_t tile = b - a.tile_begin;
+    uint32_t dir = a.inverse;
+    load_twl(a, t, twl);
+    uint64_t raw[kNttPerThread];
+    contig_fetch(a, tile, t, raw);
+#pragma unroll
+    for (int i = 0; i < kNttPerThread; i++) lds[lds_slot(t + kNttThreads * i)] = raw[i];
+    uint64_t bn = b + gridDim.x;
+    NttPass an = a;
+    uint64_t tn = 0;
+    if (bn < ntiles) {
+        an = pass_at(passes, npasses, one, bn);
+        tn = bn - an.tile_begin;
+        contig_fetch(an, tn, t, raw);
+    }
+    __syncthreads();
+    while (true) {
+        contig_rounds(lds, a, t, twl);  // ends with a barrier
+        const gptr<uint64_t> out = gmem(a.buf) + (tile << kNttTileLog) + t;
+        if (bn >= ntiles) {
+#pragma unroll
+            for (int i = 0; i < kNttPerThread; i++) out[i << 8] = lds[lds_slot(t + kNttThreads * i)];
+            break;
+        }
+        // the tile after next: its pass, looked up before this tile's stores
+        const uint64_t bn2 = bn + gridDim.x;
+        NttPass an2 = an;
+        uint64_t tn2 = 0;
+        if (bn2 < ntiles) {
+            an2 = pass_at(passes, npasses, one, bn2);
+            tn2 = bn2 - an2.tile_begin;
+        }
+        if (an.inverse != dir) {
+            load_twl(an, t, twl);  // every lane is past the rounds that read twl
+            dir = an.inverse;
+        }
+        // each lane reads and rewrites only its own slots
+#pragma unroll
+        for (int i = 0; i < kNttPerThread; i++) {
+            const uint32_t sl = lds_slot(t + kNttThreads * i);
+            const uint64_t o = lds[sl];
+            lds[sl] = raw[i];
+            out[i << 8] = o;
+        }
+        if (bn2 < ntiles) contig_fetch(an2, tn2, t, raw);
+        a = an;
+        tile = tn;
+        bn = bn2;
+        an = an2;
+        tn = tn2;
+        __syncthreads();
     }
 }
 
