@@ -174,7 +174,8 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
         uint32_t count;
         uint64_t blocks;
         int kind;       // 0 NTT, 1 Merkle
-        int hash;       // NTT: 0 strided, 1 contiguous, 2 contiguous and pipelined; Merkle: hash type
+        int hash;       // NTT: contiguous; Merkle: hash type
+        int pipe;       // NTT: ntt_pipe_kind of the list
         uint32_t tile_log;  // Merkle work-group size (log2 lanes)
     };
     std::vector<Launch> launches;
@@ -204,8 +205,7 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
             }
             if (list.empty()) continue;
             const size_t off = push(list.data(), list.size() * sizeof(NttPass));
-            const int how = contig ? (ntt_pipe_ok(list.data(), list.size()) ? 2 : 1) : 0;
-            launches.push_back({off, (uint32_t)list.size(), blocks, 0, how, 0});
+            launches.push_back({off, (uint32_t)list.size(), blocks, 0, contig, ntt_pipe_kind(list.data(), list.size()), 0});
             host.resize(align_up(host.size()));
         }
     for (int ht : {kPoseidon, kKeccak, kPoseidonBN128, kPoseidon2, kMonolith})
@@ -222,7 +222,7 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
             }
             if (list.empty()) continue;
             const size_t off = push(list.data(), list.size() * sizeof(MerkleArgs));
-            launches.push_back({off, (uint32_t)list.size(), blocks, 1, ht, tl});
+            launches.push_back({off, (uint32_t)list.size(), blocks, 1, ht, 0, tl});
             host.resize(align_up(host.size()));
         }
     size_t off0 = 0;
@@ -236,8 +236,8 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
     for (const Launch &l : launches) {
         const char *base = static_cast<const char *>(desc_.p) + l.offset;
         if (l.kind == 0) {
-            launch_ntt_list(stream_, reinterpret_cast<const NttPass *>(base), l.count, l.blocks, l.hash != 0,
-                            l.hash == 2, "batch_ntt");
+            launch_ntt_list(stream_, reinterpret_cast<const NttPass *>(base), l.count, l.blocks, l.hash == 1, l.pipe,
+                            "batch_ntt");
         } else {
             launch_merkle_pass(stream_, l.hash, reinterpret_cast<const MerkleArgs *>(base), l.count, MerkleArgs{},
                                l.blocks, l.tile_log);

@@ -41,15 +41,17 @@ uint64_t plan_dif(std::vector<NttPass> &out, uint64_t *buf, uint64_t nblk, uint3
 // stages first (the last pass contiguous).
 std::vector<uint32_t> ntt_pass_sizes(uint32_t L, const Tuning &tun);
 
-// Whether a launch over these passes can use the pipelined contiguous
-// kernel: every pass contiguous, in place, and a whole number of tiles.
-bool ntt_pipe_ok(const NttPass *host_list, size_t n);
+// Which persistent, pipelined kernel a launch over these passes can use:
+// 0 none; 1 contiguous (every pass in place and a whole number of tiles);
+// 2 + 4 * lde + (m - 1) strided single-round (every pass strided with the
+// same m <= 4 stages and the same source, in place or LDE).
+int ntt_pipe_kind(const NttPass *host_list, size_t n);
 
 // One launch over n pass descriptors in device memory (sorted by
-// tile_begin), all contiguous (s_lo == 0) or all strided; `pipe` as
-// ntt_pipe_ok of the same list.
+// tile_begin), all contiguous (s_lo == 0) or all strided; `pipe_kind` as
+// ntt_pipe_kind of the same list.
 void launch_ntt_list(hipStream_t stream, const NttPass *dev_list, uint32_t n, uint64_t blocks, bool contiguous,
-                     bool pipe, const char *name);
+                     int pipe_kind, const char *name);
 
 // table[i] = base^i, i < n.
 void fill_powers(uint64_t *dst, uint64_t n, uint64_t base, hipStream_t s);

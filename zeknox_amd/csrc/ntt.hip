@@ -15,30 +15,63 @@ namespace zkx {
 
 namespace {
 
-// Work-groups of k_ntt_contig_pipe resident at once on the current device
-// (its persistent grid).
-uint64_t pipe_groups()
+// Work-groups of a persistent kernel resident at once on the current device
+// (its grid), cached per device and kernel.
+template <class Kernel>
+uint64_t resident_groups(Kernel kernel)
 {
     static std::mutex mu;
-    static std::map<int, uint64_t> cache;
+    static std::map<std::pair<int, const void *>, uint64_t> cache;
     int dev = 0;
     ZKX_HIP(hipGetDevice(&dev));
+    const auto key = std::make_pair(dev, reinterpret_cast<const void *>(kernel));
     std::lock_guard<std::mutex> g(mu);
-    auto it = cache.find(dev);
+    auto it = cache.find(key);
     if (it != cache.end()) return it->second;
     int per_cu = 0, cus = 0;
-    ZKX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ntt_contig_pipe, kNttThreads, 0));
+    ZKX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kNttThreads, 0));
     ZKX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     const uint64_t n = (uint64_t)std::max(per_cu, 1) * (uint64_t)std::max(cus, 1);
-    cache[dev] = n;
+    cache[key] = n;
     return n;
 }
 
-void launch_contig(hipStream_t stream, const NttPass *dev_list, uint32_t n, const NttPass &one, uint64_t tiles)
+template <class Kernel>
+void launch_persistent(Kernel kernel, hipStream_t stream, const NttPass *dev_list, uint32_t n, const NttPass &one,
+                       uint64_t tiles)
 {
-    const uint64_t groups = std::min(tiles, pipe_groups());
-    hipLaunchKernelGGL(k_ntt_contig_pipe, dim3((uint32_t)groups), dim3(kNttThreads), 0, stream, dev_list, n, one,
-                       tiles);
+    const uint64_t groups = std::min(tiles, resident_groups(kernel));
+    hipLaunchKernelGGL(kernel, dim3((uint32_t)groups), dim3(kNttThreads), 0, stream, dev_list, n, one, tiles);
+}
+
+template <int MODE>
+void launch_strided_pipe(int m, hipStream_t stream, const NttPass *dev_list, uint32_t n, const NttPass &one,
+                         uint64_t tiles)
+{
+    switch (m) {
+    case 1: launch_persistent(k_ntt_strided_pipe<MODE, 1>, stream, dev_list, n, one, tiles); break;
+    case 2: launch_persistent(k_ntt_strided_pipe<MODE, 2>, stream, dev_list, n, one, tiles); break;
+    case 3: launch_persistent(k_ntt_strided_pipe<MODE, 3>, stream, dev_list, n, one, tiles); break;
+    default: launch_persistent(k_ntt_strided_pipe<MODE, 4>, stream, dev_list, n, one, tiles); break;
+    }
+}
+
+// A launch over the passes (device list, or `one` when n == 0) with the
+// kernel ntt_pipe_kind chose; false for kind 0.
+bool launch_pipe(int kind, hipStream_t stream, const NttPass *dev_list, uint32_t n, const NttPass &one,
+                 uint64_t tiles)
+{
+    if (kind == 1) {
+        launch_persistent(k_ntt_contig_pipe, stream, dev_list, n, one, tiles);
+        return true;
+    }
+    if (kind >= 2 && kind < 10) {
+        const int m = (kind - 2) % 4 + 1;
+        if (kind >= 6) launch_strided_pipe<kSrcLde>(m, stream, dev_list, n, one, tiles);
+        else launch_strided_pipe<kSrcInPlace>(m, stream, dev_list, n, one, tiles);
+        return true;
+    }
+    return false;
 }
 
 // Launch one pass (a single descriptor passed by value).
@@ -46,9 +79,8 @@ void launch_pass(Device &d, const NttPass &p, uint64_t tiles, const char *name)
 {
     if (tiles > 0x7FFFFFFFull) invalid("transform too large for one launch");
     ProfScope ps(name, d.stream);
-    if (ntt_pipe_ok(&p, 1))
-        launch_contig(d.stream, nullptr, 0u, p, tiles);
-    else if (p.s_lo == 0)
+    if (launch_pipe(ntt_pipe_kind(&p, 1), d.stream, nullptr, 0u, p, tiles)) {
+    } else if (p.s_lo == 0)
         hipLaunchKernelGGL(k_ntt_tiles<true>, dim3((uint32_t)tiles), dim3(kNttThreads), 0, d.stream,
                            (const NttPass *)nullptr, 0u, p);
     else
@@ -72,23 +104,33 @@ void run_dif(Device &d, uint64_t *buf, uint64_t nblk, uint32_t L, uint32_t lg_tw
 
 }  // namespace
 
-bool ntt_pipe_ok(const NttPass *host_list, size_t n)
+int ntt_pipe_kind(const NttPass *host_list, size_t n)
 {
+    if (n == 0) return 0;
+    const NttPass &p0 = host_list[0];
+    if (p0.s_lo == 0) {
+        for (size_t i = 0; i < n; i++) {
+            const NttPass &p = host_list[i];
+            if (p.s_lo != 0 || p.mode != kSrcInPlace || (p.total & ((1ull << kNttTileLog) - 1)) != 0) return 0;
+        }
+        return 1;
+    }
+    if (p0.m < 1 || p0.m > 4 || (p0.mode != kSrcInPlace && p0.mode != kSrcLde)) return 0;
     for (size_t i = 0; i < n; i++) {
         const NttPass &p = host_list[i];
-        if (p.s_lo != 0 || p.mode != kSrcInPlace || (p.total & ((1ull << kNttTileLog) - 1)) != 0) return false;
+        if (p.s_lo == 0 || p.m != p0.m || p.mode != p0.mode) return 0;
     }
-    return n > 0;
+    return 2 + (p0.mode == kSrcLde ? 4 : 0) + (int)(p0.m - 1);
 }
 
 void launch_ntt_list(hipStream_t stream, const NttPass *dev_list, uint32_t n, uint64_t blocks, bool contiguous,
-                     bool pipe, const char *name)
+                     int pipe_kind, const char *name)
 {
     if (blocks == 0) return;
     if (blocks > 0x7FFFFFFFull) invalid("transform batch too large for one launch");
     ProfScope ps(name, stream);
-    if (pipe)
-        launch_contig(stream, dev_list, n, NttPass{}, blocks);
+    if (launch_pipe(pipe_kind, stream, dev_list, n, NttPass{}, blocks)) {
+    }
     else if (contiguous)
         hipLaunchKernelGGL(k_ntt_tiles<true>, dim3((uint32_t)blocks), dim3(kNttThreads), 0, stream, dev_list, n,
                            NttPass{});

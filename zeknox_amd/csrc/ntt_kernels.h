@@ -508,6 +508,207 @@ k_ntt_contig_pipe(const NttPass *__restrict__ passes, uint32_t npasses, NttPass 
     }
 }
 
+// ---- strided single-round passes, persistent and pipelined
+//
+// A strided pass of m <= 4 stages is one register round (b0 = 0, RB = c =
+// 12 - m >= 8): lane t holds u = (rb << c) | o, o = t | g << 8 (rb < 2^m,
+// g < 2^(4-m)), so the twiddle of stage dd for rb is
+//   W(dd, g, v) = w_{2^(s_lo+dd+1)}^((v << s_lo) | (lo_base + o)),  v = rb mod 2^dd,
+// at most 15 values per lane, all known once the tile is known.  The next
+// tile's twiddles and input factors are loaded before this tile's stores and
+// its input words prefetched after them, so no wait covers a store issued
+// less than a tile ago or the prefetch (vector-memory counters retire in
+// order).  Per tile:
+//   round(k) | twiddles(k+1) | per position: read result(k), write
+//   input(k+1) [x factor], store result(k) | prefetch(k+2) | barrier
+// MODE is the pass's source (kSrcInPlace or kSrcLde), K its stage count; a
+// launch takes passes of one mode and one stage count (ntt_pipe_kind).
+
+// tile_step of a strided pass with m = K <= 4 stages (c = 12 - K >= 8): the
+// branches of the general form decided at compile time.
+template <int K>
+__device__ __forceinline__ uint64_t strided_step(const NttPass &a, int i)
+{
+    constexpr int cb = 4 - K;  // c - log2(256)
+    return ((uint64_t)(i & ((1 << cb) - 1)) << 8) + ((uint64_t)(i >> cb) << a.s_lo);
+}
+
+template <int K>
+constexpr int strided_ntw()
+{
+    return ((1 << K) - 1) << (4 - K);  // sum over dd of 2^dd values x 2^(4-K) groups
+}
+
+// offset of stage dd's twiddles (stages stored K-1 first, then g-major, v-minor)
+template <int K>
+constexpr int strided_tw_off(int dd)
+{
+    int off = 0;
+    for (int d = K - 1; d > dd; d--) off += (1 << d) << (4 - K);
+    return off;
+}
+
+template <int K>
+__device__ __forceinline__ void strided_tw_load(const NttPass &a, uint64_t tile, uint32_t t,
+                                                uint64_t (&w)[strided_ntw<K>()])
+{
+    const uint64_t lo_base = (tile & ((1ull << (a.s_lo - a.c)) - 1)) << a.c;
+    const cptr<const uint64_t *> tabs = cmem(a.tw_lg);
+#pragma unroll
+    for (int dd = K - 1; dd >= 0; dd--) {
+        const uint32_t lg = a.s_lo + (uint32_t)dd + 1;
+        const uint64_t *tab = tabs[lg];
+#pragma unroll
+        for (int g = 0; g < (1 << (4 - K)); g++)
+#pragma unroll
+            for (int v = 0; v < (1 << dd); v++) {
+                const uint64_t ex = ((uint64_t)v << a.s_lo) | (lo_base + (t | ((uint32_t)g << 8)));
+                w[strided_tw_off<K>(dd) + (g << dd) + v] = tw_at(tab, lg, ex, a.inverse);
+            }
+    }
+}
+
+// The round itself (ntt_round / ntt_round4 with b0 = 0 on a strided pass),
+// twiddles from registers.
+template <int K>
+__device__ __forceinline__ void strided_round(uint64_t *lds, uint32_t t, uint32_t c,
+                                              const uint64_t (&w)[strided_ntw<K>()])
+{
+    constexpr int E = kNttPerThread;
+    uint64_t x[E];
+    uint32_t sl[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const uint32_t rb = (uint32_t)e & ((1u << K) - 1), g = (uint32_t)e >> K;
+        sl[e] = lds_slot((rb << c) | t | (g << 8));
+        x[e] = lds[sl[e]];
+    }
+#pragma unroll
+    for (int dd = K - 1; dd >= 0; dd--) {
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            if ((e >> dd) & 1) continue;
+            const int f = e | (1 << dd);
+            const int rb = e & ((1 << K) - 1), g = e >> K;
+            const uint64_t wv = w[strided_tw_off<K>(dd) + (g << dd) + (rb & ((1 << dd) - 1))];
+            const uint64_t y0 = x[e], y1 = x[f];
+            x[e] = gl::add(y0, y1);
+            x[f] = gl::mul(gl::sub(y0, y1), wv);
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < E; e++) lds[sl[e]] = x[e];
+}
+
+// Row pointers of a strided tile: the buffer, and for an LDE pass the
+// coefficient row and factor row (the tile lies in one block of 2^L).
+struct StridedRows {
+    gptr<uint64_t> buf;
+    gptr<const uint64_t> src, fac;
+};
+
+template <int MODE>
+__device__ __forceinline__ StridedRows strided_rows(const NttPass &a, uint64_t tile, uint32_t t)
+{
+    const uint64_t ibase = tile_index(a, tile, t);
+    StridedRows r;
+    r.buf = gmem(a.buf) + ibase;
+    if constexpr (MODE == kSrcLde) {
+        const uint64_t blk = ibase >> a.L;
+        const uint64_t pos0 = ibase - (blk << a.L);
+        r.src = gmem(a.src) + ((blk >> a.rate) << a.L) + pos0;
+        r.fac = gmem(a.coset) + ((blk & ((1ull << a.rate) - 1)) << a.L) + pos0;
+    } else {
+        r.src = r.buf;
+        r.fac = nullptr;
+    }
+    return r;
+}
+
+template <int MODE, int K>
+__device__ __forceinline__ void strided_fetch(const NttPass &a, uint64_t tile, uint32_t t,
+                                              uint64_t (&raw)[kNttPerThread])
+{
+    const StridedRows r = strided_rows<MODE>(a, tile, t);
+#pragma unroll
+    for (int i = 0; i < kNttPerThread; i++) raw[i] = r.src[strided_step<K>(a, i)];
+}
+
+template <int MODE, int K>
+__global__ void __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_eu(kNttPipeWaves)))
+k_ntt_strided_pipe(const NttPass *__restrict__ passes, uint32_t npasses, NttPass one, uint64_t ntiles)
+{
+    __shared__ uint64_t lds[(1 << kNttTileLog) + (1 << kNttTileLog) / 32];
+    const uint32_t t = threadIdx.x;
+    const uint64_t b = blockIdx.x;
+    if (b >= ntiles) return;
+    NttPass a = pass_at(passes, npasses, one, b);
+    uint64_t tile = b - a.tile_begin;
+    uint64_t w[strided_ntw<K>()];
+    strided_tw_load<K>(a, tile, t, w);
+    uint64_t raw[kNttPerThread];
+    strided_fetch<MODE, K>(a, tile, t, raw);
+    {
+        const StridedRows r = strided_rows<MODE>(a, tile, t);
+#pragma unroll
+        for (int i = 0; i < kNttPerThread; i++) {
+            uint64_t v = raw[i];
+            if constexpr (MODE == kSrcLde) v = gl::mul(v, r.fac[strided_step<K>(a, i)]);
+            lds[lds_slot(t + kNttThreads * i)] = v;
+        }
+    }
+    uint64_t bn = b + gridDim.x;
+    NttPass an = a;
+    uint64_t tn = 0;
+    if (bn < ntiles) {
+        an = pass_at(passes, npasses, one, bn);
+        tn = bn - an.tile_begin;
+        strided_fetch<MODE, K>(an, tn, t, raw);
+    }
+    __syncthreads();
+    while (true) {
+        strided_round<K>(lds, t, a.c, w);
+        __syncthreads();
+        const StridedRows ro = strided_rows<MODE>(a, tile, t);
+        if (bn >= ntiles) {
+#pragma unroll
+            for (int i = 0; i < kNttPerThread; i++) ro.buf[strided_step<K>(a, i)] = lds[lds_slot(t + kNttThreads * i)];
+            break;
+        }
+        const uint64_t bn2 = bn + gridDim.x;
+        NttPass an2 = an;
+        uint64_t tn2 = 0;
+        if (bn2 < ntiles) {
+            an2 = pass_at(passes, npasses, one, bn2);
+            tn2 = bn2 - an2.tile_begin;
+        }
+        // the next tile's twiddles and factors, then this tile's stores
+        strided_tw_load<K>(an, tn, t, w);
+        const StridedRows rn = strided_rows<MODE>(an, tn, t);
+        uint64_t f[kNttPerThread];
+        if constexpr (MODE == kSrcLde) {
+#pragma unroll
+            for (int i = 0; i < kNttPerThread; i++) f[i] = rn.fac[strided_step<K>(an, i)];
+        }
+        // each lane reads and rewrites only its own slots
+#pragma unroll
+        for (int i = 0; i < kNttPerThread; i++) {
+            const uint32_t s = lds_slot(t + kNttThreads * i);
+            const uint64_t o = lds[s];
+            if constexpr (MODE == kSrcLde) lds[s] = gl::mul(raw[i], f[i]);
+            else lds[s] = raw[i];
+            ro.buf[strided_step<K>(a, i)] = o;
+        }
+        if (bn2 < ntiles) strided_fetch<MODE, K>(an2, tn2, t, raw);
+        a = an;
+        tile = tn;
+        bn = bn2;
+        an = an2;
+        tn = tn2;
+        __syncthreads();
+    }
+}
+
 // In-place bit reversal of every column of length 2^lg (lg >= 2*kBrTile),
 // with optional post-scaling of element at natural index t by
 // scale * (coset ? g^t : 1).  Index bits are (h: 5 | mid | l: 5); the tile for
