@@ -71,7 +71,8 @@ def test_ntt_roundtrip_on_device(zk):
     assert np.array_equal(d.to_host(), x)
 
 
-@pytest.mark.parametrize("lg,B", [(0, 3), (1, 2), (5, 7), (6, 33), (10, 135), (12, 85), (13, 2)])
+@pytest.mark.parametrize("lg,B", [(0, 3), (1, 2), (5, 7), (5, 32), (6, 33), (7, 137), (8, 400), (10, 135), (11, 271),
+                                  (12, 85), (13, 2)])
 def test_transpose_rev_vs_oracle(zk, lg, B):
     x = ob.canonical(B << lg, 9 * lg + B)
     out = np.zeros_like(x)

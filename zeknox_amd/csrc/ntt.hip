@@ -299,9 +299,43 @@ void ntt(Device &d, uint64_t *inout, uint32_t lg, uint32_t cols, bool inverse, b
                    (inverse && with_coset) ? d.coset : nullptr);
 }
 
+namespace {
+
+template <int NE>
+void launch_transpose_rows(uint32_t pw, dim3 grid, hipStream_t s, uint64_t *out, const uint64_t *in, uint32_t lg,
+                           uint32_t B, uint32_t rev, uint32_t cb)
+{
+    switch (pw) {
+    case 32: hipLaunchKernelGGL((k_transpose_rows<NE, 32>), grid, dim3(kNttThreads), 0, s, out, in, lg, B, rev, cb); break;
+    case 64: hipLaunchKernelGGL((k_transpose_rows<NE, 64>), grid, dim3(kNttThreads), 0, s, out, in, lg, B, rev, cb); break;
+    case 128: hipLaunchKernelGGL((k_transpose_rows<NE, 128>), grid, dim3(kNttThreads), 0, s, out, in, lg, B, rev, cb); break;
+    default: hipLaunchKernelGGL((k_transpose_rows<NE, 256>), grid, dim3(kNttThreads), 0, s, out, in, lg, B, rev, cb); break;
+    }
+}
+
+}  // namespace
+
 void transpose_rev(Device &d, uint64_t *out, const uint64_t *in, uint32_t lg, uint32_t cols, bool rev)
 {
     const uint64_t n = 1ull << lg;
+    if (cols >= 32 && n >= kTrRows) {
+        // whole-row tiles: columns split evenly into chunks of at most kTrMaxCols
+        const uint32_t chunks = (cols + kTrMaxCols - 1) / kTrMaxCols;
+        const uint32_t cb = (cols + chunks - 1) / chunks;
+        const uint32_t ne = (cb * kTrRows + kNttThreads - 1) / kNttThreads;
+        uint32_t pw = 32;
+        while (pw < cb) pw <<= 1;
+        const uint64_t gx = n / kTrRows;
+        if (gx > 0x7FFFFFFFull || chunks > 65535) invalid("transpose too large for one launch");
+        ProfScope ps("transpose_rev", d.stream);
+        const dim3 grid((uint32_t)gx, chunks);
+        if (ne <= 4) launch_transpose_rows<4>(pw, grid, d.stream, out, in, lg, cols, rev ? 1u : 0u, cb);
+        else if (ne <= 8) launch_transpose_rows<8>(pw, grid, d.stream, out, in, lg, cols, rev ? 1u : 0u, cb);
+        else if (ne <= 12) launch_transpose_rows<12>(pw, grid, d.stream, out, in, lg, cols, rev ? 1u : 0u, cb);
+        else launch_transpose_rows<17>(pw, grid, d.stream, out, in, lg, cols, rev ? 1u : 0u, cb);
+        ZKX_HIP(hipGetLastError());
+        return;
+    }
     const uint64_t gx = (n + kTrTile - 1) / kTrTile;
     const uint32_t gy = (cols + kTrTile - 1) / kTrTile;
     if (gx > 0x7FFFFFFFull || gy > 65535) invalid("transpose too large for one launch");

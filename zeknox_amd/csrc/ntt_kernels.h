@@ -732,6 +732,50 @@ __device__ __forceinline__ uint64_t br_post(const BitrevArgs &a, uint64_t v, uin
     return v;
 }
 
+// One lane's share of a tile pair: the 2 x 4 loads in flight before the
+// first LDS write, and no branch inside the loops (the pair / self-paired
+// case and the post-scaling are decided outside them; a branch between
+// loads would leave one load per lane in flight).
+template <bool PAIR, bool POST>
+__device__ __forceinline__ void bitrev_pair(const BitrevArgs &a, gptr<uint64_t> base, uint32_t hs, uint32_t mid,
+                                            uint32_t rmid, uint64_t (*ta)[33], uint64_t (*tb)[33])
+{
+    constexpr uint32_t T = 1u << kBrTile;
+    constexpr int PER = T * T / kNttThreads;
+    const uint32_t t = threadIdx.x;
+    uint64_t va[PER], vb[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const uint32_t e = t + kNttThreads * k, h = e >> kBrTile, l = e & (T - 1);
+        va[k] = base[((uint64_t)h << hs) | ((uint64_t)mid << kBrTile) | l];
+        if constexpr (PAIR) vb[k] = base[((uint64_t)h << hs) | ((uint64_t)rmid << kBrTile) | l];
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const uint32_t e = t + kNttThreads * k, h = e >> kBrTile, l = e & (T - 1);
+        ta[h][l] = va[k];
+        if constexpr (PAIR) tb[h][l] = vb[k];
+    }
+    __syncthreads();
+    // element (h, l) of tile(mid) goes to (rev(l), rmid, rev(h))
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const uint32_t e = t + kNttThreads * k;
+        const uint32_t oh = e >> kBrTile, ol = e & (T - 1);  // destination (h', l')
+        const uint32_t sh = brev_bits(ol, kBrTile), sl = brev_bits(oh, kBrTile);
+        const uint64_t t1 = ((uint64_t)oh << hs) | ((uint64_t)rmid << kBrTile) | ol;
+        uint64_t v1 = ta[sh][sl];
+        if constexpr (POST) v1 = br_post(a, v1, t1);
+        base[t1] = v1;
+        if constexpr (PAIR) {
+            const uint64_t t2 = ((uint64_t)oh << hs) | ((uint64_t)mid << kBrTile) | ol;
+            uint64_t v2 = tb[sh][sl];
+            if constexpr (POST) v2 = br_post(a, v2, t2);
+            base[t2] = v2;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(kNttThreads) k_bitrev_tiles(BitrevArgs a)
 {
     constexpr uint32_t T = 1u << kBrTile;
@@ -745,23 +789,13 @@ __global__ void __launch_bounds__(kNttThreads) k_bitrev_tiles(BitrevArgs a)
     if (rmid < mid) return;  // the partner tile's block does both
     const gptr<uint64_t> base = gmem(a.buf) + col * n;
     const uint32_t hs = a.lg - kBrTile;
-
-    for (uint32_t e = threadIdx.x; e < T * T; e += kNttThreads) {
-        const uint32_t h = e >> kBrTile, l = e & (T - 1);
-        ta[h][l] = base[((uint64_t)h << hs) | ((uint64_t)mid << kBrTile) | l];
-        if (rmid != mid) tb[h][l] = base[((uint64_t)h << hs) | ((uint64_t)rmid << kBrTile) | l];
-    }
-    __syncthreads();
-    // element (h, l) of tile(mid) goes to (rev(l), rmid, rev(h))
-    for (uint32_t e = threadIdx.x; e < T * T; e += kNttThreads) {
-        const uint32_t oh = e >> kBrTile, ol = e & (T - 1);  // destination (h', l')
-        const uint32_t sh = brev_bits(ol, kBrTile), sl = brev_bits(oh, kBrTile);
-        const uint64_t t1 = ((uint64_t)oh << hs) | ((uint64_t)rmid << kBrTile) | ol;
-        base[t1] = br_post(a, ta[sh][sl], t1);
-        if (rmid != mid) {
-            const uint64_t t2 = ((uint64_t)oh << hs) | ((uint64_t)mid << kBrTile) | ol;
-            base[t2] = br_post(a, tb[sh][sl], t2);
-        }
+    const bool post = a.use_scale || a.use_coset;
+    if (rmid != mid) {
+        if (post) bitrev_pair<true, true>(a, base, hs, mid, rmid, ta, tb);
+        else bitrev_pair<true, false>(a, base, hs, mid, rmid, ta, tb);
+    } else {
+        if (post) bitrev_pair<false, true>(a, base, hs, mid, rmid, ta, tb);
+        else bitrev_pair<false, false>(a, base, hs, mid, rmid, ta, tb);
     }
 }
 
@@ -807,6 +841,55 @@ __global__ void __launch_bounds__(kNttThreads) k_transpose_rev(uint64_t *__restr
             const uint64_t row = rev ? brev_bits((uint32_t)j, lg) : j;
             out[row * B + b0 + bl] = t[bl][jl];
         }
+    }
+}
+
+// The same transpose for 2^lg >= 32 rows, whole output rows at a time: a
+// tile is 32 consecutive source positions j of `cb` columns (all of B when
+// B <= 136), so each output row bitrev(j) receives one contiguous run of cb
+// words and every lane keeps NE loads in flight (NE = cb * 32 / 256 rounded
+// up; positions past the tile's columns re-read its last word, in bounds).
+// Stores: PW lanes per row (the power of two >= cb, at most 256), 256 / PW
+// rows per step.
+constexpr uint32_t kTrRows = 32;
+constexpr uint32_t kTrMaxCols = 136;
+
+template <int NE, int PW>
+__global__ void __launch_bounds__(kNttThreads) k_transpose_rows(uint64_t *__restrict__ out,
+                                                              const uint64_t *__restrict__ in, uint32_t lg,
+                                                              uint32_t B, uint32_t rev, uint32_t cb)
+{
+    static_assert(NE * kNttThreads >= kTrMaxCols * kTrRows || NE < 17, "lanes cover the tile");
+    __shared__ uint64_t tl[kTrMaxCols][kTrRows + 1];
+    const uint64_t n = 1ull << lg;
+    const uint64_t j0 = (uint64_t)blockIdx.x * kTrRows;
+    const uint32_t b0 = blockIdx.y * cb;
+    const uint32_t nc = B - b0 < cb ? B - b0 : cb;
+    const uint32_t tot = nc * kTrRows;
+    const uint32_t t = threadIdx.x;
+    const gptr<const uint64_t> src = gmem(in) + (uint64_t)b0 * n + j0;
+    uint64_t v[NE];
+#pragma unroll
+    for (int k = 0; k < NE; k++) {
+        uint32_t e = t + kNttThreads * k;
+        e = e < tot ? e : tot - 1;
+        v[k] = src[(uint64_t)(e >> 5) * n + (e & 31)];
+    }
+#pragma unroll
+    for (int k = 0; k < NE; k++) {
+        const uint32_t e = t + kNttThreads * k;
+        if (e < tot) tl[e >> 5][e & 31] = v[k];
+    }
+    __syncthreads();
+    constexpr uint32_t RPS = kNttThreads / PW;  // rows per step
+    const uint32_t r = t / PW, c = t % PW;
+    if (c >= nc) return;
+#pragma unroll
+    for (uint32_t k = 0; k < kTrRows / RPS; k++) {
+        const uint32_t jl = r + k * RPS;
+        const uint64_t j = j0 + jl;
+        const uint64_t row = rev ? brev_bits((uint32_t)j, lg) : j;
+        gmem(out)[row * B + b0 + c] = tl[c][jl];
     }
 }
 
