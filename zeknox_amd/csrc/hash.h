@@ -116,6 +116,110 @@ __device__ __forceinline__ void poseidon_permute(uint64_t s[12])
     }
 }
 
+// ------------------------------------------------- Poseidon on a lane pair
+//
+// Tree levels run at low occupancy: once the leaves are hashed, every level
+// of a tile is one permutation deep, so its time is one permutation's
+// latency on nearly idle SIMDs, and half the lanes of a level sit idle
+// anyway.  There two lanes (t, t ^ 1; h = t & 1) hash one node: lane h holds
+// the six state elements x'_m = x_{(m + 6h) mod 12}, m < 6.  The MDS is
+// circulant, so in this rotated indexing row q of lane h (state row
+// 6h + q) is sum_m CIRC[(m - q) mod 12] x'_m (+ DIAG[6h + q] x'_q): the same
+// coefficients on both lanes, except a diagonal term that differs between
+// rows q and 6 + q.  Each lane takes its partner's six words (x'_{6..11})
+// with a DPP lane swap and accumulates its own six rows: about 58% of the
+// single-lane instructions per lane.  Both lanes of a pair must be active.
+
+// The partner lane's copy of v (lanes 2k and 2k+1 exchange).
+__device__ __forceinline__ uint32_t lane_swap(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1 /* quad_perm [1,0,3,2] */, 0xF, 0xF, false);
+}
+
+// Row seeds (the next round's constants) and the MDS of the rotated state.
+__device__ __forceinline__ void poseidon_mds_pair(uint64_t x[6], const uint64_t *k, bool h)
+{
+    uint32_t lo[12], hi[12];
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+        lo[m] = gl::lo32(x[m]);
+        hi[m] = gl::hi32(x[m]);
+    }
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+        lo[6 + m] = lane_swap(lo[m]);
+        hi[6 + m] = lane_swap(hi[m]);
+    }
+#pragma unroll
+    for (int q0 = 0; q0 < 6; q0 += 2) {
+        uint64_t A[2], B[2];
+#pragma unroll
+        for (int d = 0; d < 2; d++) {
+            const uint64_t kk = h ? k[6 + q0 + d] : k[q0 + d];
+            A[d] = gl::lo32(kk);
+            B[d] = gl::hi32(kk);
+        }
+#pragma unroll
+        for (int i = 0; i < 12; i++)
+#pragma unroll
+            for (int d = 0; d < 2; d++) {
+                const int q = q0 + d, m = (i + q) % 12;
+                uint32_t c;
+                if (i == 0 && ZK_POSEIDON_MDS_DIAG[q] != ZK_POSEIDON_MDS_DIAG[6 + q]) {
+                    const uint32_t c0 = (uint32_t)(ZK_POSEIDON_MDS_CIRC[0] + ZK_POSEIDON_MDS_DIAG[q]);
+                    const uint32_t c1 = (uint32_t)(ZK_POSEIDON_MDS_CIRC[0] + ZK_POSEIDON_MDS_DIAG[6 + q]);
+                    c = h ? c1 : c0;
+                } else {
+                    c = gl::opaque((uint32_t)(ZK_POSEIDON_MDS_CIRC[i] + (i == 0 ? ZK_POSEIDON_MDS_DIAG[q] : 0)));
+                }
+                A[d] = gl::vopaque64((uint64_t)lo[m] * c + A[d]);
+                B[d] = gl::vopaque64((uint64_t)hi[m] * c + B[d]);
+            }
+#pragma unroll
+        for (int d = 0; d < 2; d++) x[q0 + d] = limb_combine(A[d], B[d]);
+    }
+}
+
+// poseidon_permute on the lane pair's rotated halves.
+__device__ __forceinline__ void poseidon_permute_pair(uint64_t x[6], bool h)
+{
+#pragma unroll
+    for (int m = 0; m < 6; m++) x[m] = gl::add(x[m], h ? c_poseidon_rc[6 + m] : c_poseidon_rc[m]);
+#pragma unroll 1
+    for (int r = 0; r < 4; r++) {
+#pragma unroll
+        for (int m = 0; m < 6; m += 2) gl::pow7_lazy2(x[m], x[m + 1]);
+        poseidon_mds_pair(x, c_poseidon_rc + 12 * (r + 1), h);
+    }
+#pragma unroll 1
+    for (int r = 4; r < 26; r++) {
+        const uint64_t s0 = gl::pow7_lazy(x[0]);  // state element 0 lives on lane 0
+        x[0] = h ? x[0] : s0;
+        poseidon_mds_pair(x, c_poseidon_rc + 12 * (r + 1), h);
+    }
+#pragma unroll 1
+    for (int r = 26; r < 30; r++) {
+#pragma unroll
+        for (int m = 0; m < 6; m += 2) gl::pow7_lazy2(x[m], x[m + 1]);
+        poseidon_mds_pair(x, c_poseidon_rc + 12 * (r + 1), h);
+    }
+}
+
+// hash_two<kPoseidon>(l, r) on a lane pair: state [l, r, 0]; the digest is
+// valid on the even lane (h = 0).
+__device__ __forceinline__ void poseidon_hash_two_pair(const uint64_t l[4], const uint64_t r[4], uint64_t out[4],
+                                                       bool h)
+{
+    uint64_t x[6];
+#pragma unroll
+    for (int m = 0; m < 4; m++) x[m] = h ? (m < 2 ? r[2 + m] : 0) : l[m];
+    x[4] = h ? 0 : r[0];
+    x[5] = h ? 0 : r[1];
+    poseidon_permute_pair(x, h);
+#pragma unroll
+    for (int i = 0; i < 4; i++) out[i] = gl::canon(x[i]);
+}
+
 // --------------------------------------------------------------- Poseidon2
 
 // External layer (lazy in and out) M_E = [[2M4, M4, M4], [M4, 2M4, M4], [M4, M4, 2M4]] with

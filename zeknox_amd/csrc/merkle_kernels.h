@@ -139,32 +139,44 @@ k_merkle_tile(const MerkleArgs *__restrict__ jobs, uint32_t njobs, MerkleArgs on
             __syncthreads();
             continue;
         }
+        // Nodes of a level: with PAIR (Poseidon) lanes 2n and 2n+1 hash node
+        // n together (a level never has more than TPB / 2 nodes) and lane 2n
+        // holds the digest; otherwise lane n hashes node n.
+        constexpr bool PAIR = HT == kPoseidon;
+        const uint32_t n = PAIR ? t >> 1 : t;
+        const bool writer = PAIR ? (t & 1) == 0 : true;
+        auto hash_node = [&](uint64_t out[4]) {
+            if constexpr (PAIR) poseidon_hash_two_pair(lds[2 * n], lds[2 * n + 1], out, (t & 1) != 0);
+            else hash_two<HT>(lds[2 * n], lds[2 * n + 1], out);
+        };
         if (lg_w == 1) {
             // width-2 top level (cur >= 2 here): caps of the subtrees in this tile
             const uint32_t pairs = cur >> 1;
-            if (t < pairs) {
+            if (n < pairs) {
                 uint64_t out[4];
-                hash_two<HT>(lds[2 * t], lds[2 * t + 1], out);
-                const uint64_t sub = (first >> 1) + t;
+                hash_node(out);
+                if (writer) {
+                    const uint64_t sub = (first >> 1) + n;
 #pragma unroll
-                for (int i = 0; i < 4; i++) gmem(a.cap)[4 * sub + i] = out[i];
+                    for (int i = 0; i < 4; i++) gmem(a.cap)[4 * sub + i] = out[i];
+                }
             }
             return;
         }
         const uint32_t half = cur >> 1;
         uint64_t out[4];
-        if (t < half) hash_two<HT>(lds[2 * t], lds[2 * t + 1], out);
+        if (n < half) hash_node(out);
         __syncthreads();
         lg_w -= 1;
         first >>= 1;
-        if (t < half) {
-            const uint64_t g = first + t;
+        if (n < half && writer) {
+            const uint64_t g = first + n;
             const uint64_t sub = g >> lg_w, idx = g & ((1ull << lg_w) - 1);
             const gptr<uint64_t> dst = node_ptr(a, sub, lg_w, idx);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 dst[i] = out[i];
-                lds[t][i] = out[i];
+                lds[n][i] = out[i];
             }
         }
         cur = half;
