@@ -327,6 +327,104 @@ __device__ __forceinline__ void poseidon2_permute(uint64_t s[12])
     for (int r = 26; r < 30; r++) external_round(r);
 }
 
+// poseidon2_permute on a lane pair (as poseidon_permute_pair: tree levels,
+// both lanes active, h = lane & 1).  External rounds keep the full state on
+// both lanes: each lane applies the S-box to its six elements (6h .. 6h+5),
+// the halves are exchanged, and both lanes run the external layer.  Internal
+// rounds update only the lane's own six elements (s_i d_i + sum, the sum
+// from both lanes' partial sums), and the halves are exchanged again before
+// the next external round.
+__device__ __forceinline__ uint64_t lane_swap64(uint64_t v)
+{
+    return gl::pack(lane_swap(gl::lo32(v)), lane_swap(gl::hi32(v)));
+}
+
+__device__ __forceinline__ void poseidon2_permute_pair(uint64_t s[12], bool h)
+{
+    p2_external(s);
+    // own[m] = s[6h + m]; the partner's half back into the full state
+    auto own_of = [&](uint64_t own[6]) {
+#pragma unroll
+        for (int m = 0; m < 6; m++) own[m] = h ? s[6 + m] : s[m];
+    };
+    auto merge = [&](const uint64_t own[6]) {
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+            const uint64_t other = lane_swap64(own[m]);
+            s[m] = h ? other : own[m];
+            s[6 + m] = h ? own[m] : other;
+        }
+    };
+    auto external_round = [&](int r) {
+        const uint64_t *rc = c_poseidon2_rc + 12 * r;
+        uint64_t own[6];
+#pragma unroll
+        for (int m = 0; m < 6; m++) own[m] = gl::add(h ? s[6 + m] : s[m], h ? rc[6 + m] : rc[m]);
+#pragma unroll
+        for (int m = 0; m < 6; m += 2) gl::pow7_lazy2(own[m], own[m + 1]);
+        merge(own);
+        p2_external(s);
+    };
+#pragma unroll 1
+    for (int r = 0; r < 4; r++) external_round(r);
+    uint64_t own[6], d[6];
+    own_of(own);
+#pragma unroll
+    for (int m = 0; m < 6; m++) d[m] = h ? ZK_POSEIDON2_DIAG[6 + m] : ZK_POSEIDON2_DIAG[m];
+#pragma unroll 1
+    for (int r = 4; r < 26; r++) {
+        const uint64_t t0 = gl::pow7_lazy(gl::add(own[0], c_poseidon2_rc[12 * r]));  // element 0: lane 0
+        own[0] = h ? own[0] : t0;
+        // sum of the lane's six (three words), plus the partner's
+        uint32_t lo = 0, hi = 0, top = 0;
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+            uint32_t c1, c2, c3;
+            lo = __builtin_addc(lo, gl::lo32(own[m]), 0u, &c1);
+            hi = __builtin_addc(hi, gl::hi32(own[m]), c1, &c2);
+            top = __builtin_addc(top, 0u, c2, &c3);
+        }
+        {
+            const uint32_t plo = lane_swap(lo), phi = lane_swap(hi), ptop = lane_swap(top);
+            uint32_t c1, c2, c3;
+            lo = __builtin_addc(lo, plo, 0u, &c1);
+            hi = __builtin_addc(hi, phi, c1, &c2);
+            top = __builtin_addc(top, ptop, c2, &c3);
+        }
+        const uint64_t sum = gl::reduce96_lazy(gl::pack(lo, hi), top);
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+            uint64_t plo, phi;
+            gl::mul128(own[m], d[m], plo, phi);
+            uint32_t c1, c2, c3, c4;
+            const uint32_t l0 = __builtin_addc(gl::lo32(plo), gl::lo32(sum), 0u, &c1);
+            const uint32_t l1 = __builtin_addc(gl::hi32(plo), gl::hi32(sum), c1, &c2);
+            const uint32_t h0 = __builtin_addc(gl::lo32(phi), 0u, c2, &c3);
+            const uint32_t h1 = __builtin_addc(gl::hi32(phi), 0u, c3, &c4);
+            own[m] = gl::reduce128_lazy(gl::pack(l0, l1), gl::pack(h0, h1));
+        }
+    }
+    merge(own);
+#pragma unroll 1
+    for (int r = 26; r < 30; r++) external_round(r);
+}
+
+// hash_two<kPoseidon2>(l, r) on a lane pair; both lanes end with the digest.
+__device__ __forceinline__ void poseidon2_hash_two_pair(const uint64_t l[4], const uint64_t r[4], uint64_t out[4],
+                                                        bool h)
+{
+    uint64_t s[12];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        s[i] = l[i];
+        s[4 + i] = r[i];
+        s[8 + i] = 0;
+    }
+    poseidon2_permute_pair(s, h);
+#pragma unroll
+    for (int i = 0; i < 4; i++) out[i] = gl::canon(s[i]);
+}
+
 template <int HT>
 __device__ __forceinline__ void permute(uint64_t s[12])
 {

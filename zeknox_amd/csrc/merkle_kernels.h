@@ -139,14 +139,15 @@ k_merkle_tile(const MerkleArgs *__restrict__ jobs, uint32_t njobs, MerkleArgs on
             __syncthreads();
             continue;
         }
-        // Nodes of a level: with PAIR (Poseidon) lanes 2n and 2n+1 hash node
-        // n together (a level never has more than TPB / 2 nodes) and lane 2n
-        // holds the digest; otherwise lane n hashes node n.
-        constexpr bool PAIR = HT == kPoseidon;
+        // Nodes of a level: with PAIR (Poseidon, Poseidon2) lanes 2n and 2n+1
+        // hash node n together (a level never has more than TPB / 2 nodes)
+        // and lane 2n writes the digest; otherwise lane n hashes node n.
+        constexpr bool PAIR = HT == kPoseidon || HT == kPoseidon2;
         const uint32_t n = PAIR ? t >> 1 : t;
         const bool writer = PAIR ? (t & 1) == 0 : true;
         auto hash_node = [&](uint64_t out[4]) {
-            if constexpr (PAIR) poseidon_hash_two_pair(lds[2 * n], lds[2 * n + 1], out, (t & 1) != 0);
+            if constexpr (HT == kPoseidon) poseidon_hash_two_pair(lds[2 * n], lds[2 * n + 1], out, (t & 1) != 0);
+            else if constexpr (HT == kPoseidon2) poseidon2_hash_two_pair(lds[2 * n], lds[2 * n + 1], out, (t & 1) != 0);
             else hash_two<HT>(lds[2 * n], lds[2 * n + 1], out);
         };
         if (lg_w == 1) {
