@@ -45,10 +45,10 @@ HBM_PEAK_GBS = 8000.0
 #   VALU wave-instructions per Poseidon permutation (SQ_INSTS_VALU / perms,
 #   leaf loads and in-tile levels included), by Merkle work-group size (log2
 #   lanes; larger tiles waste fewer lanes on the upper levels)
-VALU_WAVE_INSTS_PER_PERM = {8: 324.1, 9: 313.5, 10: 308.3}
+VALU_WAVE_INSTS_PER_PERM = {8: 314.9, 9: 310.1, 10: 307.7}
 #   HBM bytes per commit: FETCH_SIZE x 2 (the counter reads half the bytes of
 #   a plain 1 GiB copy) + WRITE_SIZE x 1, KiB x 1024, by work-group size
-LEAF_TRAFFIC_BYTES_PER_COMMIT = {8: 7.75e8, 9: 8.66e8}
+LEAF_TRAFFIC_BYTES_PER_COMMIT = {8: 7.75e8, 9: 8.83e8}
 #   NTT passes (both), same scaling
 NTT_TRAFFIC_BYTES_PER_COMMIT = 8.87e8
 # 32-bit VALU issue peak: 256 CUs x 4 SIMDs x one wave64 instruction per 4
