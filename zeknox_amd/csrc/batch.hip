@@ -192,22 +192,31 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
         ctr_ = DevBuf(std::max<size_t>(total_ctr * sizeof(uint32_t), 4096));
         ZKX_HIP(hipMemsetAsync(ctr_.p, 0, ctr_.bytes, stream_));
     }
-    for (size_t k = 0; k < max_ntt; k++)
-        for (int contig = 0; contig < 2; contig++) {
+    // NTT pass k of every job, one launch per (contiguity, pipelined kernel
+    // kind): jobs whose passes the pipelined kernels take (ntt_pipe_kind of
+    // the single pass) are not held back by others in the same launch (a
+    // launch runs pipelined only if all its passes qualify alike)
+    for (size_t k = 0; k < max_ntt; k++) {
+        std::map<std::pair<int, int>, std::vector<size_t>> groups;
+        for (size_t i = 0; i < n; i++) {
+            if (k >= ntt_plan[i].size()) continue;
+            const NttPass &p = ntt_plan[i][k];
+            groups[{p.s_lo == 0 ? 1 : 0, ntt_pipe_kind(&p, 1)}].push_back(i);
+        }
+        for (const auto &grp : groups) {
             std::vector<NttPass> list;
             uint64_t blocks = 0;
-            for (size_t i = 0; i < n; i++) {
-                if (k >= ntt_plan[i].size() || (ntt_plan[i][k].s_lo == 0) != (contig == 1)) continue;
+            for (size_t i : grp.second) {
                 NttPass p = ntt_plan[i][k];
                 p.tile_begin = blocks;
                 blocks += ntt_tiles[i];
                 list.push_back(p);
             }
-            if (list.empty()) continue;
             const size_t off = push(list.data(), list.size() * sizeof(NttPass));
-            launches.push_back({off, (uint32_t)list.size(), blocks, 0, contig, ntt_pipe_kind(list.data(), list.size()), 0});
+            launches.push_back({off, (uint32_t)list.size(), blocks, 0, grp.first.first, grp.first.second, 0});
             host.resize(align_up(host.size()));
         }
+    }
     for (int ht : {kPoseidon, kKeccak, kPoseidonBN128, kPoseidon2, kMonolith})
         for (uint32_t tl = kMerkleMinTileLog; tl <= (uint32_t)kMerkleMaxTileLog; tl++) {
             std::vector<MerkleArgs> list;
