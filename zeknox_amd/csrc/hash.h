@@ -1,6 +1,8 @@
 // Poseidon and Poseidon2 permutations (Goldilocks, width 12, rate 8,
 // 8 full + 22 partial rounds, x^7 S-box) and the plonky2 sponge, for one
-// lane.  The 12-element state lives in VGPRs.  Rounds run as a rolled loop
+// lane (the 12-element state in VGPRs), and on a pair of lanes with six
+// elements each for the tree levels (poseidon_hash_two_pair,
+// poseidon2_hash_two_pair).  Rounds run as a rolled loop
 // (the unrolled permutation is ~150 KB of code, more than the instruction
 // cache); round constants are read through the uniform round counter from
 // constant memory, i.e. as scalar loads into SGPRs.

@@ -9,7 +9,9 @@
 //
 // Schedule.  A work-group of TPB (256..1024) lanes owns a tile of consecutive
 // leaves.  Each lane hashes its leaf, then the tile is reduced level by level
-// in LDS: every level's digests are written to HBM on the way up, and a tile
+// in LDS (Poseidon and Poseidon2 on lane pairs, two lanes per node, since a
+// level is one permutation deep and runs at low occupancy): every level's
+// digests are written to HBM on the way up, and a tile
 // that reaches the width-2 top level of its subtree also writes the cap.  A
 // tile that stops below the top leaves its root for an upper pass (a tile of
 // such roots), and so on until the caps are written (no upper pass when a
