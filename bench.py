@@ -42,13 +42,20 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 --pmc on the batched 135 x 2^15 rate-3 launch (4 commits) and
 # kept in profiles/merkle_leaves_valu_135x2e18.txt and
 # profiles/hbm_traffic_135x2e15_rate3.txt:
-#   VALU wave-instructions per Poseidon permutation (SQ_INSTS_VALU / perms,
-#   leaf loads and in-tile levels included), by Merkle work-group size (log2
-#   lanes; larger tiles waste fewer lanes on the upper levels)
-VALU_WAVE_INSTS_PER_PERM = {8: 314.9, 9: 310.1, 10: 307.7}
+#   VALU and MFMA wave-instructions per Poseidon permutation (SQ_INSTS_VALU,
+#   SQ_INSTS_MFMA over permutations; leaf loads and in-tile levels included),
+#   by Merkle work-group size (log2 lanes).  The leaf permutations run their
+#   MDS layers on the matrix cores (v_mfma_i32_32x32x32_i8).
+VALU_WAVE_INSTS_PER_PERM = {8: 236.8, 9: 231.9, 10: 229.6}
+MFMA_WAVE_INSTS_PER_PERM = {8: 7.97, 9: 7.97, 10: 7.97}
+#   issue slots (one per 4 cycles) that one such MFMA takes from the vector
+#   issue of its SIMD when it runs beside full-rate VALU work (13 cycles,
+#   measured with a microbenchmark of independent chains)
+MFMA_ISSUE_SLOTS = 3.25
 #   HBM bytes per commit: FETCH_SIZE x 2 (the counter reads half the bytes of
 #   a plain 1 GiB copy) + WRITE_SIZE x 1, KiB x 1024, by work-group size
-LEAF_TRAFFIC_BYTES_PER_COMMIT = {8: 7.75e8, 9: 8.83e8}
+#   (register spills of the matrix-core path included)
+LEAF_TRAFFIC_BYTES_PER_COMMIT = {9: 1.497e9}
 #   NTT passes (both), same scaling
 NTT_TRAFFIC_BYTES_PER_COMMIT = 8.87e8
 # 32-bit VALU issue peak: 256 CUs x 4 SIMDs x one wave64 instruction per 4
@@ -457,25 +464,29 @@ def main():
     mode = "batched"
 
     # ---- roofline of the dominant kernel: the leaf-hash launch of the batch
-    # (Poseidon permutations, VALU-bound: 64-bit modular arithmetic on the
-    # 32-bit integer VALU, no MFMA).  achieved = permutations per launch x
-    # VALU lane-instructions per permutation / the launch's HIP-event duration
-    # (events on the library's stream); peak = VALU_PEAK_TOPS.
+    # (Poseidon permutations, issue-bound: 64-bit modular arithmetic on the
+    # 32-bit integer VALU, the MDS layers as i8 MFMAs that share the SIMD's
+    # issue).  achieved = permutations per launch x issue slots per
+    # permutation (VALU wave-instructions + MFMA_ISSUE_SLOTS per MFMA) x 64
+    # lanes / the launch's HIP-event duration (events on the library's
+    # stream); peak = VALU_PEAK_TOPS (one wave-instruction per 4 cycles).
     dom = max(batch_kernels, key=lambda k: batch_kernels[k]["ms_per_step"])
     leaf = batch_kernels.get("merkle_leaves")
     roofline = None
     if leaf:
         tl = schedule["merkle_tile_log"]
         perms = args.jobs * perms_per_commit(N, COLS, nc)
-        achieved = perms * VALU_WAVE_INSTS_PER_PERM[tl] * 64 / (leaf["avg_launch_ms"] * 1e-3) / 1e12
+        slots = VALU_WAVE_INSTS_PER_PERM[tl] + MFMA_ISSUE_SLOTS * MFMA_WAVE_INSTS_PER_PERM[tl]
+        achieved = perms * slots * 64 / (leaf["avg_launch_ms"] * 1e-3) / 1e12
         alg_bytes = args.jobs * (8 * COLS * N * 2 + 32 * nd)  # LDE read, leaf rows + digests written
         traffic = LEAF_TRAFFIC_BYTES_PER_COMMIT.get(tl)
-        roofline = {"bound": "valu", "achieved": round(achieved, 2), "peak": round(VALU_PEAK_TOPS, 2),
-                    "unit": "TOP/s (32-bit VALU lane-ops)", "frac": round(achieved / VALU_PEAK_TOPS, 3),
+        roofline = {"bound": "valu+mfma issue", "achieved": round(achieved, 2), "peak": round(VALU_PEAK_TOPS, 2),
+                    "unit": "T lane-issue-slots/s (32-bit VALU)", "frac": round(achieved / VALU_PEAK_TOPS, 3),
                     "traffic": round(traffic * args.jobs) if traffic else None, "algorithmic_bytes": alg_bytes,
                     "hbm_gbs": round(traffic * args.jobs / (leaf["avg_launch_ms"] * 1e-3) / 1e9, 1) if traffic else None,
                     "kernel": "merkle_leaves", "avg_launch_ms": round(leaf["avg_launch_ms"], 4),
-                    "perms_per_launch": perms, "valu_wave_insts_per_perm": VALU_WAVE_INSTS_PER_PERM[tl]}
+                    "perms_per_launch": perms, "valu_wave_insts_per_perm": VALU_WAVE_INSTS_PER_PERM[tl],
+                    "mfma_wave_insts_per_perm": MFMA_WAVE_INSTS_PER_PERM[tl], "mfma_issue_slots": MFMA_ISSUE_SLOTS}
 
     # LDE passes against HBM: algorithmic bytes of the LDE (read n, write N
     # per column) over the batch's NTT launches

@@ -66,7 +66,10 @@ def test_go_cap_kats(zk, nl, nc, cap):
 
 
 SHAPES = [(1, 7, 0), (1, 7, 1), (2, 9, 1), (3, 5, 1), (4, 2, 2), (5, 4, 3), (6, 3, 2), (7, 8, 0), (8, 16, 4),
-          (8, 17, 8), (9, 1, 4), (10, 135, 4), (11, 20, 0), (12, 80, 4), (9, 12, 9), (10, 400, 2)]
+          (8, 17, 8), (9, 1, 4), (10, 135, 4), (11, 20, 0), (12, 80, 4), (9, 12, 9), (10, 400, 2),
+          # Poseidon's whole-wave (matrix-core) leaf path: one wave of 5-element leaves, a
+          # one-element second absorb chunk, caps == leaves
+          (6, 5, 0), (6, 9, 1), (7, 6, 7)]
 
 
 @pytest.mark.parametrize("ht", [0, 1, 2, 3, 4])

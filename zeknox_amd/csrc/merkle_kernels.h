@@ -38,15 +38,18 @@
 #include "gmem.h"
 #include "hash.h"
 #include "merkle_types.h"
+#include "poseidon_mfma.h"
 
 namespace zkx {
 
-// Occupancy hint per hash type: six waves per SIMD (80 VGPRs) for Poseidon
-// (measured 3-4% faster on the 135 x 2^18 leaf launch than the compiler's
-// own 96 VGPRs at five waves) and Poseidon2 (8 VGPRs spilled outside the round
-// loop, and as fast as five waves without the spill); the others would spill
-// more, and keep the compiler's choice.
-constexpr int merkle_waves_per_eu(int ht) { return (ht == kPoseidon || ht == kPoseidon2) ? 6 : 1; }
+// Occupancy hint per hash type: four waves per SIMD (128 VGPRs) for Poseidon,
+// whose matrix-core leaf path keeps 36 VGPRs of coefficient operands and 48
+// of MFMA accumulators (measured on the 4 x 135 x 2^18 leaf launch: five
+// waves with the operands in LDS 12% slower, six waves 45% slower, both from
+// spills); six waves (80 VGPRs) for Poseidon2 (8 VGPRs spilled outside the
+// round loop, and as fast as five waves without the spill); the others would
+// spill more, and keep the compiler's choice.
+constexpr int merkle_waves_per_eu(int ht) { return ht == kPoseidon ? 4 : ht == kPoseidon2 ? 6 : 1; }
 
 // Digest slot of node `idx` of the level of width 2^lg_w in subtree `sub`.
 __device__ __forceinline__ gptr<uint64_t> node_ptr(const MerkleArgs &a, uint64_t sub, uint32_t lg_w, uint64_t idx)
@@ -65,12 +68,65 @@ __device__ __forceinline__ const MerkleArgs *find_job(const MerkleArgs *jobs, ui
     return jobs + lo;
 }
 
+// Poseidon leaf hashing by whole waves with the MDS on the matrix cores
+// (poseidon_mfma.h): wave w hashes leaves first + 64w + [0, 64), leaf
+// 32 t + j of the wave in group t, column j.  Needs whole waves (cur >= 64)
+// and leaf_size > 4; leaves are absorbed as in hash_one.  The digests land in
+// lds[leaf - first]; kt receives the round-constant table.  Every lane of the
+// work-group calls it.
+template <int TPB>
+__device__ __forceinline__ void leaf_hash_mfma(const MerkleArgs &a, uint64_t first, uint32_t cur, uint64_t (*lds)[4],
+                                               uint64_t *kt)
+{
+    const uint32_t t = threadIdx.x, w = t >> 6, j = t & 31, h = (t >> 5) & 1;
+    mfma_fill_rc(kt, t, TPB);
+    __syncthreads();
+    if (64 * w < cur) {
+        const MfmaOps o = mfma_setup();
+        const uint32_t ls = a.leaf_size;
+        const uint64_t cs = a.col_stride;
+        uint64_t s[2][6];
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+#pragma unroll
+            for (int e = 0; e < 6; e++) s[q][e] = 0;
+        for (uint32_t idx = 0; idx < ls; idx += 8) {
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const uint64_t leaf = first + 64 * w + 32 * q + j;
+                const uint32_t el0 = 6 * ((h + q) & 1);  // this lane's first element of group q
+                const gptr<const uint64_t> row = gmem(a.leaves) + leaf * a.leaf_stride;
+#pragma unroll
+                for (int e = 0; e < 6; e++) {
+                    const uint32_t k = idx + el0 + e;
+                    if (el0 + e < 8 && k < ls) {
+                        const uint64_t v = row[(uint64_t)k * cs];
+                        if (a.leaves_out) gmem(a.leaves_out)[leaf * ls + k] = v;
+                        s[q][e] = v;
+                    }
+                }
+            }
+            poseidon_permute_mfma(s, o, kt);
+        }
+        // elements 0-3 of group q sit in slots 0-3 of half q
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+            if (h == (uint32_t)q) {
+                uint64_t *d = lds[64 * w + 32 * q + j];
+#pragma unroll
+                for (int e = 0; e < 4; e++) d[e] = gl::canon(s[q][e]);
+            }
+    }
+    __syncthreads();
+}
+
 template <int HT, int TPB>
 __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(merkle_waves_per_eu(HT))))
 k_merkle_tile(const MerkleArgs *__restrict__ jobs, uint32_t njobs, MerkleArgs one)
 {
     __shared__ uint64_t lds[TPB][4];
     __shared__ uint32_t lds_last;
+    __shared__ uint64_t kt[HT == kPoseidon ? kMfmaRcWords : 1];
     const MerkleArgs *job = njobs ? find_job(jobs, njobs) : nullptr;
     const MerkleArgs a = njobs ? *job : one;
     uint32_t up = 0;  // upper passes done
@@ -80,24 +136,32 @@ k_merkle_tile(const MerkleArgs *__restrict__ jobs, uint32_t njobs, MerkleArgs on
     uint64_t first = ((uint64_t)blockIdx.x - a.tile_begin) << a.tile_log;  // first node of the tile
 
     uint64_t h[4];
+    const bool mfma = HT == kPoseidon && cur >= 64 && a.leaf_size > 4;
+    if constexpr (HT == kPoseidon)
+        if (mfma) leaf_hash_mfma<TPB>(a, first, cur, lds, kt);
     if (t < cur) {
         const uint64_t g = first + t;
         const uint64_t sub = g >> lg_w, idx = g & ((1ull << lg_w) - 1);
-        const gptr<const uint64_t> row = gmem(a.leaves) + g * a.leaf_stride;
-        const uint64_t cs = a.col_stride;
-        const gptr<uint64_t> out = a.leaves_out ? gmem(a.leaves_out) + g * a.leaf_size : nullptr;
-        hash_one<HT>(a.leaf_size,
-                     [&](uint32_t k) {
-                         const uint64_t v = row[(uint64_t)k * cs];
-                         if (out) out[k] = v;
-                         return v;
-                     },
-                     h);
+        if (mfma) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) h[i] = lds[t][i];
+        } else {
+            const gptr<const uint64_t> row = gmem(a.leaves) + g * a.leaf_stride;
+            const uint64_t cs = a.col_stride;
+            const gptr<uint64_t> out = a.leaves_out ? gmem(a.leaves_out) + g * a.leaf_size : nullptr;
+            hash_one<HT>(a.leaf_size,
+                         [&](uint32_t k) {
+                             const uint64_t v = row[(uint64_t)k * cs];
+                             if (out) out[k] = v;
+                             return v;
+                         },
+                         h);
+#pragma unroll
+            for (int i = 0; i < 4; i++) lds[t][i] = h[i];
+        }
         const gptr<uint64_t> dst = lg_w ? node_ptr(a, sub, lg_w, idx) : gmem(a.cap) + 4 * g;
 #pragma unroll
         for (int i = 0; i < 4; i++) dst[i] = h[i];
-#pragma unroll
-        for (int i = 0; i < 4; i++) lds[t][i] = h[i];
     }
     if (lg_w == 0) return;  // caps == leaves: done
     __syncthreads();
