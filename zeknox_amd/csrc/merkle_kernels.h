@@ -44,9 +44,9 @@ namespace zkx {
 
 // Occupancy hint per hash type: four waves per SIMD (128 VGPRs) for Poseidon,
 // whose matrix-core leaf path keeps 36 VGPRs of coefficient operands and 48
-// of MFMA accumulators (measured on the 4 x 135 x 2^18 leaf launch: five
-// waves with the operands in LDS 12% slower, six waves 45% slower, both from
-// spills); six waves (80 VGPRs) for Poseidon2 (8 VGPRs spilled outside the
+// of MFMA accumulators (measured on the 4 x 135 x 2^18 leaf launch: the
+// operands read from LDS inside the round loop, at five or six waves, 9%
+// slower); six waves (80 VGPRs) for Poseidon2 (8 VGPRs spilled outside the
 // round loop, and as fast as five waves without the spill); the others would
 // spill more, and keep the compiler's choice.
 constexpr int merkle_waves_per_eu(int ht) { return ht == kPoseidon ? 4 : ht == kPoseidon2 ? 6 : 1; }
