@@ -91,6 +91,11 @@ __device__ __forceinline__ void leaf_hash_mfma(const MerkleArgs &a, uint64_t fir
 #pragma unroll
             for (int e = 0; e < 6; e++) s[q][e] = 0;
         for (uint32_t idx = 0; idx < ls; idx += 8) {
+            // addresses recomputed per chunk from an opaque chunk offset:
+            // running pointers (24 VGPRs) would otherwise live through the
+            // permutation and spill
+            uint32_t kb = idx;
+            asm volatile("; chunk %0" : "+s"(kb));
 #pragma unroll
             for (int q = 0; q < 2; q++) {
                 const uint64_t leaf = first + 64 * w + 32 * q + j;
@@ -98,7 +103,7 @@ __device__ __forceinline__ void leaf_hash_mfma(const MerkleArgs &a, uint64_t fir
                 const gptr<const uint64_t> row = gmem(a.leaves) + leaf * a.leaf_stride;
 #pragma unroll
                 for (int e = 0; e < 6; e++) {
-                    const uint32_t k = idx + el0 + e;
+                    const uint32_t k = kb + el0 + e;
                     if (el0 + e < 8 && k < ls) {
                         const uint64_t v = row[(uint64_t)k * cs];
                         if (a.leaves_out) gmem(a.leaves_out)[leaf * ls + k] = v;
