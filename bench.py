@@ -54,8 +54,7 @@ MFMA_WAVE_INSTS_PER_PERM = {8: 7.97, 9: 7.97, 10: 7.97}
 MFMA_ISSUE_SLOTS = 3.25
 #   HBM bytes per commit: FETCH_SIZE x 2 (the counter reads half the bytes of
 #   a plain 1 GiB copy) + WRITE_SIZE x 1, KiB x 1024, by work-group size
-#   (register spills of the matrix-core path included)
-LEAF_TRAFFIC_BYTES_PER_COMMIT = {9: 1.497e9}
+LEAF_TRAFFIC_BYTES_PER_COMMIT = {9: 8.25e8}
 #   NTT passes (both), same scaling
 NTT_TRAFFIC_BYTES_PER_COMMIT = 8.87e8
 # 32-bit VALU issue peak: 256 CUs x 4 SIMDs x one wave64 instruction per 4
