@@ -27,8 +27,9 @@ RustError zkx_memcpy_htod(size_t device_id, void *dst, const void *src, size_t b
 RustError zkx_memcpy_dtoh(size_t device_id, void *dst, const void *src, size_t bytes);
 RustError zkx_memset(size_t device_id, void *dst, int value, size_t bytes);
 RustError zkx_device_synchronize(size_t device_id);
-/* Synthetic input on the device: dst[i] = splitmix64 of (seed, i), redrawn
- * until below p, so uniform canonical field elements (benchmarks). */
+/* Synthetic input on the device: dst[i] = the first value below p of the
+ * splitmix64 stream started at seed ^ (i * 0xD1B54A32D192ED03), so uniform
+ * canonical field elements (benchmarks; the seeded salt's stream). */
 RustError zkx_fill_random(size_t device_id, void *dst, size_t count, uint64_t seed);
 
 /* Goldilocks arithmetic on device vectors through the library's own device
@@ -38,6 +39,17 @@ RustError zkx_fill_random(size_t device_id, void *dst, size_t count, uint64_t se
  * (any a < 2^64); 5: a + b (a, b < p); 6: a - b (a, b < p); 7: a mod p.
  * No counterpart in the reference (its field tests are C++ unit tests). */
 RustError zkx_gl_ops(size_t device_id, int op, const void *a, const void *b, void *out, size_t n);
+
+/* compute_batched_lde with a reproducible salt.  compute_batched_lde seeds
+ * the salt_size columns that follow the LDE from the clock, as the reference
+ * does (ntt/ntt.cu:541-546, ntt/kernels.cu:62-84), so they differ on every
+ * call; here element g of the salt block (g < salt_size * 2^(lg+rate),
+ * column-major like the LDE) is the first value below p of the splitmix64
+ * stream started at salt_seed ^ (g * 0xD1B54A32D192ED03), the stream of
+ * zkx_fill_random.  Everything else is compute_batched_lde.  No counterpart
+ * in the reference. */
+RustError zkx_compute_batched_lde_seeded(size_t device_id, void *output, void *input, uint32_t lg_domain_size,
+                                         NTT_Direction ntt_direction, NTT_Config cfg, uint64_t salt_seed);
 
 /* The stream the library enqueues its work on (a hipStream_t). */
 void *zkx_stream(size_t device_id);
@@ -97,6 +109,16 @@ RustError zkx_set_autotune(size_t device_id, int on);
  * built-in default), top stages first; returns the pass count (at most
  * `cap` written), -1 for parameters out of range.  Host-only. */
 int zkx_ntt_pass_sizes(uint32_t lg, const ZkxTuning *tuning, uint32_t *sizes, int cap);
+/* The pass plan of the one-launch Merkle forest over nl leaves and nc caps
+ * with work-groups of at most 2^tile_log lanes (8..10), host-only: per pass
+ * (at most `cap` written) its input nodes over all subtrees, log2 width of
+ * its input level per subtree and tile log, and for every pass after the
+ * first the index of its first arrival counter (the last work-group of a
+ * group of 2^tile_log tiles to arrive hashes the group's roots); *counters =
+ * the arrival counters the launch uses.  Returns the pass count, -1 for
+ * sizes that are not a linear forest. */
+int zkx_merkle_plan(uint64_t nl, uint64_t nc, uint32_t tile_log, uint64_t *count, uint32_t *lg_w,
+                    uint32_t *tile_logs, uint32_t *first_ctr, int cap, uint64_t *counters);
 /* Forget every shape entry and restore the built-in default. */
 RustError zkx_tuning_clear(size_t device_id);
 /* The shape table as text, one shape per line. */

@@ -690,6 +690,42 @@ void orc_fill_canonical(u64 *out, u64 count, u64 seed)
     }
 }
 
+/* Seeded salt / zkx_fill_random stream: element g is the first value below p
+ * of the splitmix64 sequence (Steele, Lea and Flood 2014; the published
+ * mixer with increment 0x9E3779B97F4A7C15) started at seed ^ (g *
+ * 0xD1B54A32D192ED03).  The reference's salt is clock-seeded curand
+ * (ntt/kernels.cu:62-84, ntt.cu:541-546) and reproducible by nothing, so
+ * this seeded form is the library's own definition; the mixer is pinned by
+ * splitmix64's published outputs. */
+void orc_salt(u64 *out, u64 count, u64 seed)
+{
+    for (u64 g = 0; g < count; g++) {
+        u64 x = seed ^ (g * 0xD1B54A32D192ED03ull), z;
+        do {
+            x += 0x9E3779B97F4A7C15ull;
+            z = x;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            z ^= z >> 31;
+        } while (z >= GL_P);
+        out[g] = z;
+    }
+}
+
+/* The raw splitmix64 outputs of `seed` (no rejection): pins the mixer
+ * against the published sequence. */
+void orc_splitmix64(u64 *out, u64 count, u64 seed)
+{
+    u64 x = seed;
+    for (u64 i = 0; i < count; i++) {
+        x += 0x9E3779B97F4A7C15ull;
+        u64 z = x;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        out[i] = z ^ (z >> 31);
+    }
+}
+
 int orc_num_threads(void)
 {
 #ifdef _OPENMP

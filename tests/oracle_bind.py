@@ -40,6 +40,7 @@ def _load_oracle():
         ("orc_lde", None, [_u64p, _u64p, u32, u32, u32, i, i, u64]),
         ("orc_transpose_rev", None, [_u64p, _u64p, u32, u32]),
         ("orc_fill_canonical", None, [_u64p, u64, u64]),
+        ("orc_salt", None, [_u64p, u64, u64]), ("orc_splitmix64", None, [_u64p, u64, u64]),
         ("orc_num_threads", i, []),
     ]:
         fn = getattr(lib, name)
@@ -136,3 +137,16 @@ def transpose_rev(data, lg_n, batches):
 
 def bitrev(x, bits):
     return int(format(x, f"0{bits}b")[::-1], 2) if bits else 0
+
+
+def salt(count, seed):
+    """The seeded salt / zkx_fill_random stream: element g from its own seed."""
+    out = np.empty(int(count), dtype=np.uint64)
+    orc.orc_salt(_ptr(out), out.size, seed)
+    return out
+
+
+def splitmix64(count, seed):
+    out = np.empty(int(count), dtype=np.uint64)
+    orc.orc_splitmix64(_ptr(out), out.size, seed)
+    return out

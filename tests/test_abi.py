@@ -114,3 +114,12 @@ def test_tuning_out_of_range_rejected():
         zk.ntt_pass_sizes(20, zk.Tuning(10, 0, 8))
     with pytest.raises(zk.ZeknoxError):
         zk.ntt_pass_sizes(20, zk.Tuning(9, 2, 8))
+
+
+def test_init_cuda_degree_reports_bad_degree():
+    """init_cuda_degree returns nothing (lib.h:28): an invalid degree is
+    reported through zkx_last_error instead of being ignored."""
+    import zeknox_amd as zk
+    with pytest.raises(zk.ZeknoxError):
+        zk.init_cuda_degree_rs(40)
+    assert "max_degree" in zk.last_error()

@@ -234,3 +234,34 @@ def test_transpose_rev_brute_force():
     for t in range(n):
         for b in range(B):
             assert out[t * B + b] == x[b * n + ob.bitrev(t, lg)]
+
+
+# splitmix64 (Steele, Lea, Flood 2014): outputs published for these seeds --
+# 1234567 (the splitmix64 test sequence), 1477776061723855037 (the reference
+# vectors of the rand_xoshiro crate, made with Vigna's splitmix64.c) and 0.
+SPLITMIX64_KATS = [
+    (1234567, [6457827717110365317, 3203168211198807973, 9817491932198370423, 4593380528125082431,
+               16408922859458223821]),
+    (1477776061723855037, [1985237415132408290, 2979275885539914483, 13511426838097143398]),
+    (0, [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4]),
+]
+
+
+@pytest.mark.parametrize("seed,outs", SPLITMIX64_KATS)
+def test_splitmix64_published_outputs(seed, outs):
+    assert [int(v) for v in ob.splitmix64(len(outs), seed)] == outs
+
+
+def test_seeded_salt_stream_structure():
+    """orc_salt: element g is the first splitmix64 output below p of the
+    sequence seeded with seed ^ (g * 0xD1B54A32D192ED03); any prefix of a
+    longer fill is the shorter fill."""
+    seed, n = 0x5A17, 300
+    s = ob.salt(n, seed)
+    assert np.array_equal(ob.salt(57, seed), s[:57])
+    assert all(int(v) < P for v in s)
+    for g in (0, 1, 2, 77, 299):
+        start = (seed ^ (g * 0xD1B54A32D192ED03)) & (2**64 - 1)
+        first = next(int(v) for v in ob.splitmix64(8, start) if int(v) < P)
+        assert int(s[g]) == first
+    assert len(set(int(v) for v in s)) == n

@@ -151,6 +151,10 @@ def _load():
         "zkx_tuning_save": (RustError, [sz, ctypes.c_char_p]),
         "zkx_tuning_load": (RustError, [sz, ctypes.c_char_p]),
         "zkx_ntt_pass_sizes": (ctypes.c_int, [u32, ctypes.POINTER(Tuning), ctypes.POINTER(u32), ctypes.c_int]),
+        "zkx_merkle_plan": (ctypes.c_int, [u64, u64, u32, ctypes.POINTER(u64), ctypes.POINTER(u32),
+                                           ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.c_int,
+                                           ctypes.POINTER(u64)]),
+        "zkx_compute_batched_lde_seeded": (RustError, [sz, vp, vp, u32, ctypes.c_int, NTTConfig, u64]),
         "zkx_device_synchronize": (RustError, [sz]),
         "zkx_stream": (vp, [sz]),
         "zkx_profile_enable": (RustError, [ctypes.c_int]),
@@ -260,12 +264,25 @@ def list_devices_info_rs():
     _check(lib.list_devices_info())
 
 
+def _check_last(what):
+    e = last_error()
+    if e:
+        raise ZeknoxError(1, f"{what}: {e}")
+
+
 def init_cuda_rs():
     lib.init_cuda()
+    _check_last("init_cuda")
 
 
 def init_cuda_degree_rs(max_degree):
     lib.init_cuda_degree(max_degree)
+    _check_last("init_cuda_degree")
+
+
+def init_cuda_degree_and_generator_rs(max_degree, generator):
+    lib.init_cuda_degree_and_generator(max_degree, generator)
+    _check_last("init_cuda_degree_and_generator")
 
 
 def init_twiddle_factors_rs(device_id, lg_n):
@@ -278,6 +295,13 @@ def init_coset_rs(device_id, lg_n, coset_gen):
 
 def lde_batch(device_id, output, input, log_n_size, cfg, direction=FORWARD):
     _check(lib.compute_batched_lde(device_id, _addr(output), _addr(input), log_n_size, direction, cfg))
+
+
+def lde_batch_seeded(device_id, output, input, log_n_size, cfg, salt_seed, direction=FORWARD):
+    """compute_batched_lde whose salt columns come from `salt_seed`
+    (zkx_compute_batched_lde_seeded)."""
+    _check(lib.zkx_compute_batched_lde_seeded(device_id, _addr(output), _addr(input), log_n_size, direction, cfg,
+                                              salt_seed))
 
 
 def lde_batch_multi_gpu(output, input, num_gpu, cfg, log_n_size, total_in, total_out):
@@ -429,6 +453,19 @@ def ntt_pass_sizes(lg, tuning=None):
     if n < 0:
         raise ZeknoxError(1, "schedule parameters out of range")
     return list(out[:n])
+
+
+def merkle_plan(num_leaves, num_caps, tile_log):
+    """The one-launch Merkle forest's pass plan (host-only): a list of
+    (nodes, lg_w, tile_log, first_counter) per pass and the counter total."""
+    n = 8
+    count = (ctypes.c_uint64 * n)()
+    lg_w, tl, fc = (ctypes.c_uint32 * n)(), (ctypes.c_uint32 * n)(), (ctypes.c_uint32 * n)()
+    total = ctypes.c_uint64()
+    k = lib.zkx_merkle_plan(num_leaves, num_caps, tile_log, count, lg_w, tl, fc, n, ctypes.byref(total))
+    if k < 0:
+        raise ZeknoxError(1, "not a linear Merkle forest")
+    return [(count[i], lg_w[i], tl[i], fc[i]) for i in range(k)], total.value
 
 
 # ---- commit server -----------------------------------------------------------

@@ -58,6 +58,9 @@ public:
                 uint64_t *const *lde_bufs = nullptr);
     std::mutex mu;
     void sync();
+    // After a failed commit or wait: the fused Merkle launches' arrival
+    // counters may hold counts, so the next commit zeroes them first.
+    void mark_failed() { ctr_dirty_ = true; }
     // Wait for the engine's work, then free its working buffer (it grows
     // back on the next commit that needs it).
     void trim();
@@ -73,6 +76,7 @@ private:
     DevBuf scratch_;           // LDE of every job, DIF order
     DevBuf desc_;              // descriptor lists (device)
     DevBuf ctr_;               // arrival counters of the fused Merkle passes (zero between launches)
+    bool ctr_dirty_ = false;   // ctr_ must be zeroed before the next launch
     void *pinned_ = nullptr;   // descriptor staging (host, pinned)
     size_t pinned_bytes_ = 0;
     hipEvent_t staged_ = nullptr;  // last staging copy

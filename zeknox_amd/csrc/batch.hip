@@ -190,7 +190,11 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
     if (ctr_.bytes < total_ctr * sizeof(uint32_t)) {
         sync();
         ctr_ = DevBuf(std::max<size_t>(total_ctr * sizeof(uint32_t), 4096));
+        ctr_dirty_ = true;
+    }
+    if (ctr_dirty_ && ctr_.p) {
         ZKX_HIP(hipMemsetAsync(ctr_.p, 0, ctr_.bytes, stream_));
+        ctr_dirty_ = false;
     }
     // NTT pass k of every job, one launch per (contiguity, pipelined kernel
     // kind): jobs whose passes the pipelined kernels take (ntt_pipe_kind of
@@ -242,15 +246,20 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
     // half of a batch's NTT passes beside the first half's hashing on a
     // second stream was measured 7% slower: the co-resident NTT work-groups
     // take the hash kernel's occupancy)
-    for (const Launch &l : launches) {
-        const char *base = static_cast<const char *>(desc_.p) + l.offset;
-        if (l.kind == 0) {
-            launch_ntt_list(stream_, reinterpret_cast<const NttPass *>(base), l.count, l.blocks, l.hash == 1, l.pipe,
-                            "batch_ntt");
-        } else {
-            launch_merkle_pass(stream_, l.hash, reinterpret_cast<const MerkleArgs *>(base), l.count, MerkleArgs{},
-                               l.blocks, l.tile_log);
+    try {
+        for (const Launch &l : launches) {
+            const char *base = static_cast<const char *>(desc_.p) + l.offset;
+            if (l.kind == 0) {
+                launch_ntt_list(stream_, reinterpret_cast<const NttPass *>(base), l.count, l.blocks, l.hash == 1,
+                                l.pipe, "batch_ntt");
+            } else {
+                launch_merkle_pass(stream_, l.hash, reinterpret_cast<const MerkleArgs *>(base), l.count,
+                                   MerkleArgs{}, l.blocks, l.tile_log);
+            }
         }
+    } catch (...) {
+        ctr_dirty_ = true;
+        throw;
     }
 }
 

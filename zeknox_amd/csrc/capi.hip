@@ -12,6 +12,7 @@
 
 #include "../../include/zeknox.h"
 #include "merkle.h"
+#include "merkle_cpu.h"
 #include "ntt.h"
 
 using namespace zkx;
@@ -59,7 +60,10 @@ void copy_out(void *user, bool on_device, const uint64_t *dev, size_t elems, Dev
         ZKX_HIP(hipMemcpyAsync(user, dev, elems * sizeof(uint64_t), hipMemcpyDeviceToHost, d.stream));
 }
 
-void lde_entry(size_t device_id, void *output, void *input, uint32_t lg_n, NTT_Direction dir, NTT_Config cfg)
+// salt_seed: null for the reference's clock-seeded salt (ntt.cu:541-546),
+// else the seed of a reproducible salt.
+void lde_entry(size_t device_id, void *output, void *input, uint32_t lg_n, NTT_Direction dir, NTT_Config cfg,
+               const uint64_t *salt_seed = nullptr)
 {
     if (lg_n == 0 || cfg.extension_rate_bits < 1) invalid("invalid args");
     if (!output || !input) invalid("null buffer");
@@ -74,7 +78,8 @@ void lde_entry(size_t device_id, void *output, void *input, uint32_t lg_n, NTT_D
     if (cfg.batches) lde(d, out.p, in.p, lg_n, cfg.extension_rate_bits, cfg.batches, dir == inverse, cfg.with_coset);
     if (cfg.salt_size) {
         const uint64_t count = (uint64_t)cfg.salt_size << lg;
-        const uint64_t seed = (uint64_t)time(nullptr) * 0x9E3779B97F4A7C15ull ^ (uint64_t)clock();
+        const uint64_t seed =
+            salt_seed ? *salt_seed : (uint64_t)time(nullptr) * 0x9E3779B97F4A7C15ull ^ (uint64_t)clock();
         fill_salt(out.p + ((size_t)cfg.batches << lg), count, seed, d.stream);
     }
     copy_out(output, cfg.are_outputs_on_device, out.p, n_out, d);
@@ -82,7 +87,7 @@ void lde_entry(size_t device_id, void *output, void *input, uint32_t lg_n, NTT_D
 }
 
 void merkle_entry(void *dig, void *cap, const void *leaves, uint64_t nd, uint64_t nc, uint64_t nl, uint64_t ls,
-                  uint64_t h, uint64_t ht, uint64_t gpu_id, bool host_buffers)
+                  uint64_t h, uint64_t ht, uint64_t gpu_id)
 {
     MerkleSpec s;
     s.num_digests = nd;
@@ -99,28 +104,20 @@ void merkle_entry(void *dig, void *cap, const void *leaves, uint64_t nd, uint64_
     check_merkle(s);
     Device &d = device(gpu_id);
     std::lock_guard<std::mutex> g(d.mu);
-    if (!host_buffers) {
+    try {
         merkle_build(d, s);
         d.sync();
-        return;
+    } catch (...) {
+        d.merkle_ctr_dirty = true;  // a failed launch may leave arrival counts behind
+        throw;
     }
-    const bool direct = nc == nl;
-    DevBuf dl(nl * ls * sizeof(uint64_t)), dc(nc * 4 * sizeof(uint64_t)), dd(direct ? 0 : nd * 4 * sizeof(uint64_t));
-    ZKX_HIP(hipMemcpyAsync(dl.p, leaves, dl.bytes, hipMemcpyHostToDevice, d.stream));
-    s.leaves = dl.as<uint64_t>();
-    s.caps = dc.as<uint64_t>();
-    s.digests = direct ? nullptr : dd.as<uint64_t>();
-    merkle_build(d, s);
-    ZKX_HIP(hipMemcpyAsync(cap, dc.p, dc.bytes, hipMemcpyDeviceToHost, d.stream));
-    if (!direct && dd.bytes) ZKX_HIP(hipMemcpyAsync(dig, dd.p, dd.bytes, hipMemcpyDeviceToHost, d.stream));
-    d.sync();
 }
 
 void merkle_void(void *dig, void *cap, const void *leaves, uint64_t nd, uint64_t nc, uint64_t nl, uint64_t ls,
-                 uint64_t h, uint64_t ht, uint64_t gpu_id, bool host_buffers)
+                 uint64_t h, uint64_t ht, uint64_t gpu_id)
 {
     try {
-        merkle_entry(dig, cap, leaves, nd, nc, nl, ls, h, ht, gpu_id, host_buffers);
+        merkle_entry(dig, cap, leaves, nd, nc, nl, ls, h, ht, gpu_id);
         set_last_error("");
     } catch (const std::exception &e) {
         set_last_error(e.what());
@@ -183,9 +180,27 @@ RustError init_coset(size_t device_id, size_t lg_domain_size, const uint64_t cos
 
 void init_cuda_degree_and_generator(const uint32_t max_degree, const uint64_t group_generator)
 {
+    // lib.cu:137-155: coset table of 2^max_degree powers and the twiddle
+    // tables of every log size 2..max_degree on every device.  The function
+    // returns nothing, so a failure is recorded for zkx_last_error and
+    // reported on stderr; the first one stops the initialisation.
+    auto check = [](RustError e) {
+        if (e.code == 0) return true;
+        const std::string m = e.message ? e.message : "init_cuda: unknown error";
+        free(const_cast<char *>(e.message));
+        set_last_error(m);
+        fprintf(stderr, "zeknox: init_cuda: %s\n", m.c_str());
+        return false;
+    };
+    set_last_error("");
+    if (max_degree > 32) {
+        check(err(1, "init_cuda: max_degree exceeds 32"));
+        return;
+    }
     for (int i = 0; i < device_count(); i++) {
-        init_coset((size_t)i, max_degree, group_generator);
-        for (uint32_t k = 2; k <= max_degree; k++) init_twiddle_factors((size_t)i, k);
+        if (!check(init_coset((size_t)i, max_degree, group_generator))) return;
+        for (uint32_t k = 2; k <= max_degree; k++)
+            if (!check(init_twiddle_factors((size_t)i, k))) return;
     }
 }
 
@@ -215,6 +230,12 @@ RustError compute_batched_lde(size_t device_id, void *output, void *input, uint3
                               NTT_Direction ntt_direction, NTT_Config cfg)
 {
     return guarded([&] { lde_entry(device_id, output, input, lg_domain_size, ntt_direction, cfg); });
+}
+
+RustError zkx_compute_batched_lde_seeded(size_t device_id, void *output, void *input, uint32_t lg_domain_size,
+                                         NTT_Direction ntt_direction, NTT_Config cfg, uint64_t salt_seed)
+{
+    return guarded([&] { lde_entry(device_id, output, input, lg_domain_size, ntt_direction, cfg, &salt_seed); });
 }
 
 RustError compute_batched_lde_multi_gpu(void *output, void *input, uint32_t num_gpu, NTT_Direction ntt_direction,
@@ -254,7 +275,7 @@ void fill_digests_buf_linear_gpu_with_gpu_ptr(void *digests_buf_gpu_ptr, void *c
                                               uint64_t cap_height, uint64_t hash_type, uint64_t gpu_id)
 {
     merkle_void(digests_buf_gpu_ptr, cap_buf_gpu_ptr, leaves_buf_gpu_ptr, digests_buf_size, cap_buf_size,
-                leaves_buf_size, leaf_size, cap_height, hash_type, gpu_id, false);
+                leaves_buf_size, leaf_size, cap_height, hash_type, gpu_id);
 }
 
 void fill_digests_buf_linear_multigpu_with_gpu_ptr(void *digests_buf_gpu_ptr, void *cap_buf_gpu_ptr,
@@ -263,15 +284,23 @@ void fill_digests_buf_linear_multigpu_with_gpu_ptr(void *digests_buf_gpu_ptr, vo
                                                    uint64_t leaf_size, uint64_t cap_height, uint64_t hash_type)
 {
     merkle_void(digests_buf_gpu_ptr, cap_buf_gpu_ptr, leaves_buf_gpu_ptr, digests_buf_size, cap_buf_size,
-                leaves_buf_size, leaf_size, cap_height, hash_type, 0, false);
+                leaves_buf_size, leaf_size, cap_height, hash_type, 0);
 }
 
 void fill_digests_buf_linear_cpu(void *digests_buf_ptr, void *cap_buf_ptr, const void *leaves_buf_ptr,
                                  uint64_t digests_buf_size, uint64_t cap_buf_size, uint64_t leaves_buf_size,
                                  uint64_t leaf_size, uint64_t cap_height, uint64_t hash_type)
 {
-    merkle_void(digests_buf_ptr, cap_buf_ptr, leaves_buf_ptr, digests_buf_size, cap_buf_size, leaves_buf_size,
-                leaf_size, cap_height, hash_type, 0, true);
+    // host buffers, host threads: no HIP device is used or needed
+    try {
+        cpu::fill_digests_linear(static_cast<uint64_t *>(digests_buf_ptr), static_cast<uint64_t *>(cap_buf_ptr),
+                                 static_cast<const uint64_t *>(leaves_buf_ptr), digests_buf_size, cap_buf_size,
+                                 leaves_buf_size, leaf_size, cap_height, hash_type);
+        set_last_error("");
+    } catch (const std::exception &e) {
+        set_last_error(e.what());
+        fprintf(stderr, "zeknox: fill_digests_buf_linear_cpu: %s\n", e.what());
+    }
 }
 
 }  // extern "C"
@@ -312,8 +341,13 @@ RustError zkx_commit_batch(size_t device_id, const ZkxCommitJob *jobs, size_t nj
         Device &d = device(device_id);
         Engine &e = default_engine(d);
         std::lock_guard<std::mutex> g(e.mu);
-        e.commit(v.data(), v.size());
-        e.sync();
+        try {
+            e.commit(v.data(), v.size());
+            e.sync();
+        } catch (...) {
+            e.mark_failed();
+            throw;
+        }
     });
 }
 
@@ -384,6 +418,36 @@ int zkx_ntt_pass_sizes(uint32_t lg, const ZkxTuning *t, uint32_t *sizes, int cap
         const std::vector<uint32_t> p = ntt_pass_sizes(lg, v);
         for (size_t i = 0; i < p.size() && (int)i < cap; i++) sizes[i] = p[i];
         return (int)p.size();
+    } catch (...) {
+        return -1;
+    }
+}
+
+int zkx_merkle_plan(uint64_t nl, uint64_t nc, uint32_t tile_log, uint64_t *count, uint32_t *lg_w, uint32_t *tile_logs,
+                    uint32_t *first_ctr, int cap, uint64_t *counters)
+{
+    try {
+        MerkleSpec s;
+        s.num_leaves = nl;
+        s.num_caps = nc;
+        s.cap_height = nc ? (uint64_t)(63 - __builtin_clzll(nc)) : 0;
+        s.num_digests = nc == nl ? 0 : 2 * (nl - nc);
+        s.leaf_size = 8;
+        // the plan needs sizes only: placeholder buffers pass the checks
+        s.leaves = s.caps = s.digests = reinterpret_cast<uint64_t *>(256);
+        check_merkle(s);
+        std::vector<MerkleArgs> passes;
+        plan_merkle(s, passes, tile_log);
+        MerkleArgs f;
+        const uint64_t n = fuse_merkle_passes(passes, f);
+        for (size_t i = 0; i < passes.size() && (int)i < cap; i++) {
+            count[i] = passes[i].count;
+            lg_w[i] = passes[i].lg_w;
+            tile_logs[i] = passes[i].tile_log;
+            first_ctr[i] = i ? f.next_ctr[i - 1] : 0;
+        }
+        if (counters) *counters = n;
+        return (int)passes.size();
     } catch (...) {
         return -1;
     }

@@ -64,6 +64,7 @@ struct DevBuf {
 struct Device {
     int id = 0;
     hipStream_t stream = nullptr;  // blocking stream: ordered after the null stream
+    std::once_flag stream_once;    // creates `stream` on the first device() call
     hipDeviceProp_t prop{};
     std::mutex mu;                 // serialises table builds and compute calls
 
@@ -79,8 +80,11 @@ struct Device {
     std::map<std::tuple<uint32_t, uint32_t, bool, uint64_t>, uint64_t *> lde_factors;
 
     // Zeroed arrival counters of the fused Merkle launches on the device
-    // stream (each launch leaves them zero); grows on demand, under mu.
+    // stream (each completed launch leaves them zero); grows on demand, under
+    // mu.  A launch that fails part-way may leave counts behind: the caller
+    // then sets merkle_ctr_dirty and the next merkle_counters() zeroes them.
     DevBuf merkle_ctr;
+    bool merkle_ctr_dirty = false;
     uint32_t *merkle_counters(uint64_t n);
     // One device-side descriptor slot for those launches (written with a
     // stream-ordered copy before each launch), under mu.

@@ -66,10 +66,9 @@ Device &device(size_t id)
                                     std::to_string(r.devs.size()) + " devices)");
     Device &d = *r.devs[i];
     ZKX_HIP(hipSetDevice(d.id));
-    if (!d.stream) {
-        std::lock_guard<std::mutex> g(d.mu);
-        if (!d.stream) ZKX_HIP(hipStreamCreate(&d.stream));
-    }
+    // once per device, without holding d.mu (a compute call may hold it);
+    // a throwing creation leaves the flag unset for the next caller
+    std::call_once(d.stream_once, [&] { ZKX_HIP(hipStreamCreate(&d.stream)); });
     return d;
 }
 
@@ -81,7 +80,11 @@ uint32_t *Device::merkle_counters(uint64_t n)
     if (merkle_ctr.bytes < bytes) {
         sync();  // no launch may still use the old counters
         merkle_ctr = DevBuf(std::max<size_t>(bytes, 4096));
+        merkle_ctr_dirty = true;
+    }
+    if (merkle_ctr_dirty) {
         ZKX_HIP(hipMemsetAsync(merkle_ctr.p, 0, merkle_ctr.bytes, stream));
+        merkle_ctr_dirty = false;
     }
     return merkle_ctr.as<uint32_t>();
 }

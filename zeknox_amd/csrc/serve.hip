@@ -245,7 +245,12 @@ bool Server::launch(size_t e, std::vector<ServeRequest *> &batch, std::string &e
         }
         {
             std::lock_guard<std::mutex> g(eng.mu);
-            eng.commit(jobs.data(), jobs.size(), nullptr, nullptr, lde.data());
+            try {
+                eng.commit(jobs.data(), jobs.size(), nullptr, nullptr, lde.data());
+            } catch (...) {
+                eng.mark_failed();
+                throw;
+            }
         }
         ZKX_HIP(hipEventRecord(slot.done, st));
         const double t = now();
@@ -278,6 +283,8 @@ void Server::run(size_t e)
             if (rc != hipSuccess) {
                 ok = false;
                 err = hipGetErrorString(rc);
+                std::lock_guard<std::mutex> ge(engines_[e]->mu);
+                engines_[e]->mark_failed();
             }
         }
         g.lock();
