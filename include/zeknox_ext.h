@@ -51,6 +51,25 @@ RustError zkx_gl_ops(size_t device_id, int op, const void *a, const void *b, voi
 RustError zkx_compute_batched_lde_seeded(size_t device_id, void *output, void *input, uint32_t lg_domain_size,
                                          NTT_Direction ntt_direction, NTT_Config cfg, uint64_t salt_seed);
 
+/* fill_digests_buf_linear_gpu_with_gpu_ptr (same arguments, results and
+ * checks) without any device allocation, for callers that cannot allocate:
+ * code under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  No counterpart in the reference.  The fused launch's scratch
+ * (a pass descriptor and the upper passes' arrival counters) is a fixed
+ * ZKX_NOALLOC_SCRATCH_BYTES array per device, held until the process ends.
+ * It holds the arrival counters of every tree of up to 2^32 leaves at any
+ * work-group size (at most 65,793 of its 131,008); a tree that needs more is
+ * rejected with code 1, as are trees beyond the fused launch's four passes
+ * (also rejected by fill_digests_buf_linear_gpu_with_gpu_ptr).  Every call zeroes
+ * the counters it uses and returns when its work is complete; calls on one
+ * device are serialised by the library, so they never overlap on the array.
+ * On failure the message is malloc'd (free() it), as for RustError. */
+#define ZKX_NOALLOC_SCRATCH_BYTES (512u * 1024u)
+RustError zkx_fill_digests_buf_linear_gpu_noalloc(void *digests_buf, void *cap_buf, void *leaves_buf,
+                                                  uint64_t digests_buf_size, uint64_t cap_buf_size,
+                                                  uint64_t leaves_buf_size, uint64_t leaf_size, uint64_t cap_height,
+                                                  uint64_t hash_type, uint64_t gpu_id);
+
 /* The stream the library enqueues its work on (a hipStream_t). */
 void *zkx_stream(size_t device_id);
 

@@ -133,6 +133,7 @@ def _load():
         "fill_digests_buf_linear_gpu_with_gpu_ptr": (None, [vp, vp, vp, u64, u64, u64, u64, u64, u64, u64]),
         "fill_digests_buf_linear_multigpu_with_gpu_ptr": (None, [vp, vp, vp, u64, u64, u64, u64, u64, u64]),
         "fill_digests_buf_linear_cpu": (None, [vp, vp, vp, u64, u64, u64, u64, u64, u64]),
+        "zkx_fill_digests_buf_linear_gpu_noalloc": (RustError, [vp, vp, vp, u64, u64, u64, u64, u64, u64, u64]),
         "cuda_available": (ctypes.c_bool, []),
         "zkx_last_error": (ctypes.c_char_p, []),
         "zkx_device_malloc": (RustError, [sz, ctypes.POINTER(vp), sz]),
@@ -330,6 +331,15 @@ def fill_digests_buf_linear_gpu_with_gpu_ptr(digests, caps, leaves, digests_size
     e = last_error()
     if e:
         raise ZeknoxError(1, e)
+
+
+def fill_digests_buf_linear_gpu_noalloc(digests, caps, leaves, digests_size, caps_size, leaves_size,
+                                        leaf_size, cap_height, hash_type, gpu_id=0):
+    """fill_digests_buf_linear_gpu_with_gpu_ptr without device allocation
+    (zkx_fill_digests_buf_linear_gpu_noalloc); raises ZeknoxError."""
+    _check(lib.zkx_fill_digests_buf_linear_gpu_noalloc(_addr(digests), _addr(caps), _addr(leaves), digests_size,
+                                                       caps_size, leaves_size, leaf_size, cap_height, hash_type,
+                                                       gpu_id))
 
 
 def fill_digests_buf_linear_cpu(digests, caps, leaves, digests_size, caps_size, leaves_size, leaf_size,

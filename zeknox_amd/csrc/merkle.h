@@ -21,12 +21,15 @@ struct MerkleSpec {
 // Throws Error(1, ...) when the sizes are not a linear Merkle forest.
 void check_merkle(const MerkleSpec &s);
 // Pass descriptors of one tree, in launch order (tile_begin left 0), with
-// tiles of at most 2^tile_log_max nodes (8..10).
+// tiles of at most 2^tile_log_max nodes (8..10); appended to `passes`, or
+// written to a fixed array (returns the pass count; no heap use).
 void plan_merkle(const MerkleSpec &s, std::vector<MerkleArgs> &passes, uint32_t tile_log_max);
+uint32_t plan_merkle(const MerkleSpec &s, MerkleArgs (&passes)[kMerkleMaxPasses], uint32_t tile_log_max);
 // One descriptor that runs every pass of the tree in one launch (the upper
 // passes fused into the first); returns the arrival counters it needs
 // (its `ctr` is left null for the caller to point at that many zeroed words).
 uint64_t fuse_merkle_passes(const std::vector<MerkleArgs> &passes, MerkleArgs &out);
+uint64_t fuse_merkle_passes(const MerkleArgs *passes, uint32_t n, MerkleArgs &out);
 // One launch over `n` job descriptors in device memory (n > 0), or over the
 // single descriptor `one` (n == 0); `blocks` work-groups of 2^tile_log_max
 // lanes in total (every descriptor's tile_log <= tile_log_max).

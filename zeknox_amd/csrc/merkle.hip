@@ -33,7 +33,7 @@ void check_merkle(const MerkleSpec &s)
     if (!s.digests) invalid("null digest buffer");
 }
 
-void plan_merkle(const MerkleSpec &s, std::vector<MerkleArgs> &passes, uint32_t tile_log_max)
+uint32_t plan_merkle(const MerkleSpec &s, MerkleArgs (&passes)[kMerkleMaxPasses], uint32_t tile_log_max)
 {
     if (tile_log_max < (uint32_t)kMerkleMinTileLog || tile_log_max > (uint32_t)kMerkleMaxTileLog)
         invalid("Merkle tile size out of range");
@@ -49,36 +49,52 @@ void plan_merkle(const MerkleSpec &s, std::vector<MerkleArgs> &passes, uint32_t 
         a.count = s.num_leaves;
         a.lg_w = 0;
         a.tile_log = std::min<uint32_t>(tile_log_max, log2u(a.count));
-        passes.push_back(a);
-        return;
+        passes[0] = a;
+        return 1;
     }
     const uint32_t h = log2u(s.num_caps);
     uint32_t lg_w = log2u(s.num_leaves) - h;  // per-subtree width of the input level
     a.sdl = s.num_digests >> h;
+    uint32_t n = 0;
     while (true) {
+        if (n == (uint32_t)kMerkleMaxPasses) invalid("tree too large");
         a.count = s.num_caps << lg_w;
         a.lg_w = lg_w;
         a.tile_log = std::min<uint32_t>(tile_log_max, log2u(a.count));
-        passes.push_back(a);
+        passes[n++] = a;
         if (lg_w <= a.tile_log) break;  // this pass reaches the caps
         lg_w -= a.tile_log;
     }
+    return n;
 }
 
-uint64_t fuse_merkle_passes(const std::vector<MerkleArgs> &passes, MerkleArgs &out)
+void plan_merkle(const MerkleSpec &s, std::vector<MerkleArgs> &passes, uint32_t tile_log_max)
 {
-    if (passes.empty() || passes.size() > (size_t)kMerkleMaxPasses) invalid("internal: Merkle pass plan");
+    MerkleArgs p[kMerkleMaxPasses];
+    const uint32_t n = plan_merkle(s, p, tile_log_max);
+    passes.insert(passes.end(), p, p + n);
+}
+
+uint64_t fuse_merkle_passes(const MerkleArgs *passes, uint32_t n, MerkleArgs &out)
+{
+    if (n == 0 || n > (uint32_t)kMerkleMaxPasses) invalid("internal: Merkle pass plan");
     out = passes[0];
-    out.n_next = (uint32_t)passes.size() - 1;
+    out.n_next = n - 1;
     out.ctr = nullptr;
     uint64_t ctrs = 0;
-    for (size_t p = 1; p < passes.size(); p++) {
+    for (uint32_t p = 1; p < n; p++) {
         out.next_tile_log[p - 1] = passes[p].tile_log;
         out.next_ctr[p - 1] = (uint32_t)ctrs;
         ctrs += passes[p].count >> passes[p].tile_log;
     }
     if (ctrs > 0xFFFFFFFFull) invalid("tree too large");
     return ctrs;
+}
+
+uint64_t fuse_merkle_passes(const std::vector<MerkleArgs> &passes, MerkleArgs &out)
+{
+    if (passes.size() > (size_t)kMerkleMaxPasses) invalid("internal: Merkle pass plan");
+    return fuse_merkle_passes(passes.data(), (uint32_t)passes.size(), out);
 }
 
 void launch_merkle_pass(hipStream_t stream, int hash_type, const MerkleArgs *dev_list,
@@ -105,10 +121,10 @@ void merkle_build(Device &d, const MerkleSpec &s)
 {
     check_merkle(s);
     const uint32_t tl = d.tuning_default().merkle_tile_log;
-    std::vector<MerkleArgs> passes;
-    plan_merkle(s, passes, tl);
+    MerkleArgs passes[kMerkleMaxPasses];
+    const uint32_t np = plan_merkle(s, passes, tl);
     MerkleArgs f;
-    const uint64_t nctr = fuse_merkle_passes(passes, f);
+    const uint64_t nctr = fuse_merkle_passes(passes, np, f);
     f.tile_begin = 0;
     if (!nctr) {
         launch_merkle_pass(d.stream, (int)s.hash_type, nullptr, 0, f, f.count >> f.tile_log, tl);
