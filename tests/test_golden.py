@@ -135,3 +135,46 @@ def test_gpu_plain_headline_commit_matches_reference_fixture(zk):
     d_dig, d_cap = zk.DeviceBuffer(nd * 4), zk.DeviceBuffer(nc * 4)
     zk.fill_digests_buf_linear_gpu_with_gpu_ptr(d_dig, d_cap, d_leaves, nd, nc, N, cols, h, ht, 0)
     check_commit(rec, d_leaves.to_host(), d_dig.to_host(nd * 4), d_cap.to_host())
+
+
+# ------------------------------------- benchmark workloads (workloads.json)
+
+WL = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "workloads.json")))
+
+
+def _oracle_commit_sha(coeffs, cols, lg_n, rate, ht):
+    """SHA-256 of the oracle's leaf rows, digests and caps, and the first cap."""
+    leaves = ob.transpose_rev(ob.lde(coeffs, lg_n, rate, cols), lg_n + rate, cols)
+    dig, cap = ob.merkle(leaves, cols, 4, ht)
+    return (sha(leaves), sha(dig), sha(cap)), [str(int(v)) for v in cap[:4]]
+
+
+def test_workload_jobs_match_their_generators():
+    """The fixture's job lists are the ones zeknox_amd.workloads draws."""
+    from zeknox_amd.workloads import mixed_batch_jobs, served_trace
+    assert [(r["cols"], r["lg_n"], r["rate"], r["hash"], r["seed"]) for r in WL["batch64"]] == mixed_batch_jobs()
+    s = WL["served"]
+    tr = served_trace(s["seed"], s["seconds"], s["rate_hz"])
+    assert [(r["cols"], r["lg_n"], r["rate"], r["hash"]) for r in s["requests"]] == [t[1:] for t in tr]
+
+
+@pytest.mark.parametrize("j", [j for j, r in enumerate(WL["batch64"]) if r["lg_n"] + r["rate"] <= 13][:6])
+def test_oracle_matches_batch64_fixture(j):
+    """Small jobs of the mixed batch recomputed by the oracle alone (its Merkle
+    forest, not the reference build that wrote the fixture)."""
+    r = WL["batch64"][j]
+    got, cap0 = _oracle_commit_sha(ob.salt(r["cols"] << r["lg_n"], r["seed"]), r["cols"], r["lg_n"], r["rate"],
+                                   r["hash"])
+    assert got == (r["leaves_sha256"], r["digests_sha256"], r["caps_sha256"]) and cap0 == r["cap0"]
+
+
+def test_oracle_matches_served_fixture():
+    """The smallest requests of the served trace (both hash types), prefixes
+    of one seeded buffer, recomputed by the oracle."""
+    s = WL["served"]
+    reqs = sorted((r for r in s["requests"] if "leaves_sha256" in r), key=lambda r: r["cols"] << (r["lg_n"] + r["rate"]))
+    picked = [next(r for r in reqs if r["hash"] == h) for h in (0, 3)]
+    pool = ob.salt(max(r["cols"] << r["lg_n"] for r in picked), s["seed"])
+    for r in picked:
+        got, cap0 = _oracle_commit_sha(pool[: r["cols"] << r["lg_n"]], r["cols"], r["lg_n"], r["rate"], r["hash"])
+        assert got == (r["leaves_sha256"], r["digests_sha256"], r["caps_sha256"]) and cap0 == r["cap0"]

@@ -507,8 +507,9 @@ class Server:
         _check(lib.zkx_server_wait_any(self.h, ctypes.byref(rid), ctypes.byref(r)))
         return rid.value, r
 
-    def results(self, r, cols, lg_n, rate, cap_height):
-        """Copy a finished request's outputs to the host: (leaves, digests, caps)."""
+    def results(self, r, cols, lg_n, rate, cap_height, leaves=True):
+        """Copy a finished request's outputs to the host: (leaves, digests,
+        caps); leaves None if not kept or not asked for."""
         N = 1 << (lg_n + rate)
         nc = 1 << cap_height
         nd = 0 if nc == N else 2 * (N - nc)
@@ -518,8 +519,8 @@ class Server:
             if n:
                 _check(lib.zkx_memcpy_dtoh(self.device_id, out.ctypes.data, ptr, n * 8))
             return out
-        leaves = fetch(r.leaves, cols * N) if r.leaves else None
-        return leaves, fetch(r.digests, nd * 4), fetch(r.caps, nc * 4)
+        rows = fetch(r.leaves, cols * N) if (r.leaves and leaves) else None
+        return rows, fetch(r.digests, nd * 4), fetch(r.caps, nc * 4)
 
     def release(self, rid):
         _check(lib.zkx_server_release(self.h, rid))
