@@ -38,7 +38,8 @@
  *   fill_digests             merkle/merkle.cpp:19-161 (linear forest layout)
  *                            pinned: Go caps KATs, reference CPU (oracle/_ref)
  *   ntt / lde                ntt/ntt.cu:106-158,253-309,477-564; kernels.cu:14-212
- *                            (natural-order coset NTT, zero-padded LDE)
+ *                            (natural-order coset NTT, zero-padded LDE; its
+ *                            field product uses the Goldilocks identities)
  *                            pinned: brute-force O(n^2) DFT, closed forms,
  *                            inverse-of-forward identity, Horner spot checks
  *   transpose_rev            ntt/kernels.cu:94-135; wrappers/rust/tests/utils.rs:15-27
@@ -588,37 +589,80 @@ int orc_fill_digests(u64 *dig, u64 *cap, const u64 *leaves, u64 nd, u64 nc, u64 
 
 /* -------------------------------------------------------------------- ntt */
 
+/* the low `bits` bits of x in reverse order (bits <= 32) */
 static unsigned bitrev(u64 x, unsigned bits)
 {
-    u64 r = 0;
-    for (unsigned i = 0; i < bits; i++) r |= ((x >> i) & 1) << (bits - 1 - i);
-    return (unsigned)r;
+    uint32_t v = (uint32_t)x;
+    v = ((v >> 1) & 0x55555555u) | ((v & 0x55555555u) << 1);
+    v = ((v >> 2) & 0x33333333u) | ((v & 0x33333333u) << 2);
+    v = ((v >> 4) & 0x0F0F0F0Fu) | ((v & 0x0F0F0F0Fu) << 4);
+    v = ((v >> 8) & 0x00FF00FFu) | ((v & 0x00FF00FFu) << 8);
+    v = (v >> 16) | (v << 16);
+    return bits ? v >> (32 - bits) : 0u;
+}
+
+/* Field product with the Goldilocks identities 2^64 = 2^32 - 1 and
+ * 2^96 = -1 (mod p) (ff/goldilocks.hpp:164-177 reduce128) for the NTT's
+ * inner loop; equal to orc_gl_mul, only faster than a 128-bit division. */
+static inline u64 ntt_mul(u64 a, u64 b)
+{
+    const u128 x = (u128)a * b;
+    const u64 lo = (u64)x, hi = (u64)(x >> 64);
+    const u64 h0 = hi & 0xFFFFFFFFull, h1 = hi >> 32;
+    const u64 t = (lo - h1) - (((u64)0 - (u64)(lo < h1)) & 0xFFFFFFFFull); /* borrow: lo - h1 + p */
+    const u64 u = h0 * 0xFFFFFFFFull;
+    u64 r = t + u;
+    r += ((u64)0 - (u64)(r < t)) & 0xFFFFFFFFull; /* carry: 2^64 = 2^32 - 1 */
+    return r - (((u64)0 - (u64)(r >= GL_P)) & GL_P);
+}
+
+static inline u64 ntt_add(u64 a, u64 b) /* a, b < p */
+{
+    const u64 s = a + b;
+    return s - (((u64)0 - (u64)((s < a) | (s >= GL_P))) & GL_P);
+}
+
+static inline u64 ntt_sub(u64 a, u64 b) /* a, b < p */
+{
+    return (a - b) + (((u64)0 - (u64)(a < b)) & GL_P);
+}
+
+/* Per-stage twiddles of a 2^lg transform: stage s (butterfly span 2^s) uses
+ * w_{2^s}^k, k < 2^(s-1), at tw[2^(s-1) - 1 + k] (w or w^-1). */
+static u64 *ntt_twiddles(unsigned lg, int inverse)
+{
+    u64 *tw = (u64 *)malloc(((1ull << lg) > 1 ? (1ull << lg) : 2) * sizeof(u64));
+    for (unsigned s = 1; s <= lg; s++) {
+        u64 half = 1ull << (s - 1), *t = tw + half - 1;
+        u64 wm = orc_gl_omega(s);
+        if (inverse) wm = orc_gl_inv(wm);
+        t[0] = 1;
+        for (u64 k = 1; k < half; k++) t[k] = ntt_mul(t[k - 1], wm);
+    }
+    return tw;
 }
 
 /* In-place natural-order -> natural-order DFT of size 2^lg over one column:
  * out[j] = sum_i in[i] * w^(i*j), w = OMEGA[lg] (or its inverse).
- * Bit-reverse + iterative radix-2 Cooley-Tukey. */
-static void ntt_column(u64 *a, unsigned lg, int inverse)
+ * Bit-reverse + iterative radix-2 Cooley-Tukey; tw from ntt_twiddles. */
+static void ntt_column(u64 *a, unsigned lg, const u64 *tw)
 {
     u64 n = 1ull << lg;
+    for (u64 i = 0; i < n; i++)
+        if (a[i] >= GL_P) a[i] -= GL_P; /* ntt_add / ntt_sub take canonical operands */
     for (u64 i = 0; i < n; i++) {
         u64 j = bitrev(i, lg);
         if (j > i) { u64 t = a[i]; a[i] = a[j]; a[j] = t; }
     }
     for (unsigned s = 1; s <= lg; s++) {
         u64 m = 1ull << s, half = m >> 1;
-        u64 wm = orc_gl_omega(s);
-        if (inverse) wm = orc_gl_inv(wm);
-        u64 *tw = (u64 *)malloc(half * sizeof(u64));
-        tw[0] = 1;
-        for (u64 k = 1; k < half; k++) tw[k] = orc_gl_mul(tw[k - 1], wm);
+        const u64 *t = tw + half - 1;
         for (u64 base = 0; base < n; base += m)
             for (u64 k = 0; k < half; k++) {
-                u64 u = a[base + k], v = orc_gl_mul(a[base + k + half], tw[k]);
-                a[base + k] = orc_gl_add(u, v);
-                a[base + k + half] = orc_gl_sub(u, v);
+                u64 u = a[base + k], v = ntt_mul(a[base + k + half], t[k]);
+                a[base + k] = ntt_add(u, v);
+                a[base + k + half] = ntt_sub(u, v);
             }
-        free(tw);
     }
 }
 
@@ -629,22 +673,24 @@ static void ntt_column(u64 *a, unsigned lg, int inverse)
 void orc_ntt(u64 *data, unsigned lg, unsigned batches, int inverse, int coset, u64 gen)
 {
     u64 n = 1ull << lg;
+    u64 *tw = ntt_twiddles(lg, inverse);
 #pragma omp parallel for schedule(dynamic)
     for (long long b = 0; b < (long long)batches; b++) {
         u64 *a = data + b * n;
         if (!inverse && coset) {
             u64 g = 1;
-            for (u64 i = 0; i < n; i++) { a[i] = orc_gl_mul(a[i], g); g = orc_gl_mul(g, gen); }
+            for (u64 i = 0; i < n; i++) { a[i] = ntt_mul(a[i], g); g = ntt_mul(g, gen); }
         }
-        ntt_column(a, lg, inverse);
+        ntt_column(a, lg, tw);
         if (inverse) {
             u64 ninv = orc_gl_inv(n % GL_P), g = 1;
             for (u64 i = 0; i < n; i++) {
-                if (coset) { a[i] = orc_gl_mul(a[i], g); g = orc_gl_mul(g, gen); }
-                a[i] = orc_gl_mul(a[i], ninv);
+                if (coset) { a[i] = ntt_mul(a[i], g); g = ntt_mul(g, gen); }
+                a[i] = ntt_mul(a[i], ninv);
             }
         }
     }
+    free(tw);
 }
 
 /* compute_batched_lde semantics (ntt.cu:477-564): per column zero-pad 2^lg_n
@@ -654,7 +700,8 @@ void orc_lde(u64 *out, const u64 *in, unsigned lg_n, unsigned rate, unsigned bat
              int inverse, int coset, u64 gen)
 {
     u64 n = 1ull << lg_n, N = n << rate;
-    for (u64 b = 0; b < batches; b++) {
+#pragma omp parallel for schedule(static)
+    for (long long b = 0; b < (long long)batches; b++) {
         memcpy(out + b * N, in + b * n, n * sizeof(u64));
         memset(out + b * N + n, 0, (N - n) * sizeof(u64));
     }
