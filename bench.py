@@ -46,21 +46,48 @@ HBM_PEAK_GBS = 8000.0
 #   SQ_INSTS_MFMA over permutations; leaf loads and in-tile levels included),
 #   by Merkle work-group size (log2 lanes).  The leaf permutations run their
 #   MDS layers on the matrix cores (v_mfma_i32_32x32x32_i8).
-VALU_WAVE_INSTS_PER_PERM = {8: 236.8, 9: 231.9, 10: 229.6}
-MFMA_WAVE_INSTS_PER_PERM = {8: 7.97, 9: 7.97, 10: 7.97}
+#   keyed by (merkle_tile_log, merkle_valu)
+VALU_WAVE_INSTS_PER_PERM = {(8, 0): 236.80, (9, 0): 231.97, (10, 0): 229.57, (9, 1): 310.03}
+MFMA_WAVE_INSTS_PER_PERM = {(8, 0): 7.969, (9, 0): 7.969, (10, 0): 7.969, (9, 1): 0.0}
+#   SHA-256 (first 16 hex digits) of the leaf kernel's sources those counts
+#   were measured on (leaf_kernel_digest()); a build whose sources differ is
+#   flagged in the JSON ("counters_match_build": false)
+LEAF_COUNTERS_SOURCES = "6877227349fca0b4"
 #   issue slots (one per 4 cycles) that one such MFMA takes from the vector
-#   issue of its SIMD when it runs beside full-rate VALU work (13 cycles,
-#   measured with a microbenchmark of independent chains)
+#   issue of its SIMD beside full-rate VALU work: measured live by the
+#   "issue" leg (zkx_issue_probe); this value only when that leg is skipped
 MFMA_ISSUE_SLOTS = 3.25
 #   HBM bytes per commit: FETCH_SIZE x 2 (the counter reads half the bytes of
 #   a plain 1 GiB copy) + WRITE_SIZE x 1, KiB x 1024, by work-group size
-LEAF_TRAFFIC_BYTES_PER_COMMIT = {9: 8.25e8}
+LEAF_TRAFFIC_BYTES_PER_COMMIT = {(8, 0): 8.184e8, (9, 0): 8.222e8, (10, 0): 8.196e8, (9, 1): 8.629e8}
+# Sources of the leaf-hash launch (zeknox_amd/csrc)
+LEAF_KERNEL_SOURCES = ("gl64.h", "gl_constants.h", "gmem.h", "hash.h", "poseidon_mfma.h", "merkle_kernels.h",
+                       "merkle_launch.h", "merkle_launch_impl.h", "merkle_types.h", "merkle_poseidon.hip")
 #   NTT passes (both), same scaling
 NTT_TRAFFIC_BYTES_PER_COMMIT = 8.87e8
+#   VALU wave-instructions of the NTT launches per commit (both passes,
+#   SQ_INSTS_VALU), keyed by the LDE's pass split (stages per pass), measured
+#   on the sources NTT_COUNTERS_SOURCES
+NTT_VALU_WAVE_INSTS_PER_COMMIT = {(3, 12): 1.4536e8, (4, 11): 1.4735e8}
+NTT_COUNTERS_SOURCES = "a90afaf1241ae432"
 # 32-bit VALU issue peak: 256 CUs x 4 SIMDs x one wave64 instruction per 4
 # cycles x 2.4 GHz = 39.3 T lane-ops/s (MI355X's published 78.6 TFLOP/s
 # vector FP64 counts an FMA as 2 flops on the same issue rate)
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+
+
+NTT_KERNEL_SOURCES = ("gl64.h", "gmem.h", "ntt_kernels.h", "ntt_types.h")
+
+
+def leaf_kernel_digest(names=LEAF_KERNEL_SOURCES):
+    """SHA-256 (first 16 hex digits) of a kernel's sources (by default the
+    leaf-hash launch's)."""
+    import hashlib
+    h = hashlib.sha256()
+    for name in names:
+        with open(os.path.join(ROOT, "zeknox_amd", "csrc", name), "rb") as f:
+            h.update(name.encode() + b"\0" + f.read())
+    return h.hexdigest()[:16]
 
 
 def parse():
@@ -68,20 +95,29 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--jobs", type=int, default=4, help="135 x 2^15 rate-3 commits per batched step")
+    ap.add_argument("--jobs", type=int, default=16, help="135 x 2^15 rate-3 commits per batched step")
+    ap.add_argument("--no-graphs", action="store_true", help="batched steps launched kernel by kernel "
+                    "(no captured graph replay)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="served-stream duration (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=40.0, help="served-stream arrivals per second")
+    ap.add_argument("--serve-streams", type=int, default=3, help="commit engines (streams) of the server")
     ap.add_argument("--serve-trace", default=None,
                     help="write one CSV line per served request (times in ms from the first arrival)")
     ap.add_argument("--no-tune", action="store_true", help="batched path on the default schedule (no autotuning)")
-    ap.add_argument("--schedule", default=None, metavar="S,E,T",
-                    help="batched 135 x 2^15 schedule (ntt_strided_max,ntt_even,merkle_tile_log) instead of autotuning")
+    ap.add_argument("--schedule", default=None, metavar="S,E,T[,V]",
+                    help="batched 135 x 2^15 schedule (ntt_strided_max,ntt_even,merkle_tile_log[,merkle_valu]) "
+                         "instead of autotuning")
     ap.add_argument("--no-extra", action="store_true", help="skip the 64-job and 8 x 400 x 2^20 batches")
-    ap.add_argument("--legs", default="batch64,large,hashers,ntt",
-                    help="extra legs to run: batch64, large, hashers, ntt (comma-separated)")
+    ap.add_argument("--legs", default="batch64,large,hashers,ntt,config1",
+                    help="extra legs to run (comma-separated): batch64, large, hashers, ntt, config1 (by "
+                         "default); sweep (served-stream arrival-rate sweep), cpu3 and cpu4 (CPU baselines of the "
+                         "64-job batch and of one 400 x 2^20 Poseidon2 job) on request")
     ap.add_argument("--extra-steps", type=int, default=3, help="timed steps of those batches")
     ap.add_argument("--cpu-runs", type=int, default=5, help="whole CPU commits timed (median reported)")
+    ap.add_argument("--no-issue", action="store_true", help="skip the issue-rate probe (roofline uses constants)")
+    ap.add_argument("--sweep-rates", default="40,80,120,160,200", help="arrival rates of the sweep leg (per s)")
+    ap.add_argument("--sweep-seconds", type=float, default=20.0, help="duration of each sweep point")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain elementwise copy of MIB MiB three times (scale for HBM counters)")
     return ap.parse_args()
@@ -133,6 +169,176 @@ def cpu_baseline(np, coeffs, runs, gpu_caps):
                       + ")",
             "caps_equal_gpu": same, "nproc": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
             "cpu_model": cpu_model, "runs_s": [round(t, 3) for t in tot]}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            return next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        return ""
+
+
+def _ref_merkle(np, ob, ref, leaves, cols, nl, ht):
+    """fill_digests_buf_linear_cpu of the reference build (oracle/_ref), or the
+    oracle's restatement where that build is absent: (digests, caps)."""
+    nc = 1 << CAP_H
+    nd = 2 * (nl - nc)
+    if ref is None:
+        return ob.merkle(leaves, cols, CAP_H, ht)
+    dig = np.zeros(nd * 4, np.uint64)
+    cap = np.zeros(nc * 4, np.uint64)
+    ref.fill_digests_buf_linear_cpu(ob._ptr(dig), ob._ptr(cap), ob._ptr(leaves), nd, nc, nl, cols, CAP_H, ht)
+    return dig, cap
+
+
+def cpu_config1(np, runs, lg=13, cols=135, seed=13):
+    """Config 1 on the host cores: the Poseidon Merkle forest of 2^13 leaves x
+    135 (seeded canonical elements), cap height 4, through the reference's own
+    fill_digests_buf_linear_cpu (native/merkle/merkle.cpp:126-161, OpenMP),
+    `runs` times, median reported."""
+    import statistics
+    import oracle_bind as ob
+    ref = ob.ref_lib()
+    nl = 1 << lg
+    leaves = ob.canonical(cols << lg, seed)
+    ts, caps = [], None
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        _, caps = _ref_merkle(np, ob, ref, leaves, cols, nl, 0)
+        ts.append(time.perf_counter() - t0)
+    med = statistics.median(ts)
+    return {"value": round(1.0 / med, 2), "unit": "trees/s", "ms_per_tree": round(med * 1e3, 2),
+            "kind": "reference" if ref is not None else "port", "cores": ob.orc.orc_num_threads(),
+            "sample": f"{runs} Poseidon Merkle forests of 2^{lg} leaves x {cols}, cap_height {CAP_H}, median",
+            "runs_ms": [round(t * 1e3, 2) for t in ts], "caps": caps, "leaves": leaves}
+
+
+def gpu_config1(zk, np, steps, cpu_caps, leaves, lg=13, cols=135):
+    """Config 1 on the GPU and on the library's own host path: the same
+    2^13 x 135 Poseidon forest through fill_digests_buf_linear_gpu_with_gpu_ptr
+    (device pointers), zkx_fill_digests_buf_linear_gpu_noalloc, and the
+    library's fill_digests_buf_linear_cpu (host buffers, no GPU); caps compared
+    with the reference CPU's."""
+    nl, nc = 1 << lg, 1 << CAP_H
+    nd = 2 * (nl - nc)
+    dl = zk.DeviceBuffer.from_host(leaves)
+    dd, dc = zk.DeviceBuffer(nd * 4), zk.DeviceBuffer(nc * 4)
+    gpu = _timed(zk, lambda: zk.fill_digests_buf_linear_gpu_with_gpu_ptr(dd, dc, dl, nd, nc, nl, cols, CAP_H, 0, 0),
+                 steps, 2)
+    same_gpu = bool(np.array_equal(dc.to_host(), cpu_caps))
+    na = _timed(zk, lambda: zk.fill_digests_buf_linear_gpu_noalloc(dd, dc, dl, nd, nc, nl, cols, CAP_H, 0, 0),
+                steps, 2)
+    same_na = bool(np.array_equal(dc.to_host(), cpu_caps))
+    hd, hc = np.zeros(nd * 4, np.uint64), np.zeros(nc * 4, np.uint64)
+    t0 = time.perf_counter()
+    for _ in range(max(steps // 4, 3)):
+        zk.fill_digests_buf_linear_cpu(hd, hc, leaves, nd, nc, nl, cols, CAP_H, 0)
+    host = (time.perf_counter() - t0) / max(steps // 4, 3)
+    for b in (dl, dd, dc):
+        b.free()
+    return {"workload": f"config 1: Poseidon Merkle forest of 2^{lg} leaves x {cols}, cap_height {CAP_H}",
+            "gpu_ms_per_tree": round(gpu * 1e3, 4), "gpu_trees_per_s": round(1.0 / gpu, 1),
+            "gpu_noalloc_ms_per_tree": round(na * 1e3, 4),
+            "library_host_ms_per_tree": round(host * 1e3, 2), "library_host_threads": os.cpu_count(),
+            "caps_equal_reference_cpu": same_gpu and same_na and bool(np.array_equal(hc, cpu_caps))}
+
+
+def cpu_config3(np, runs):
+    """Config 3 on the host cores: the 64 mixed jobs (zeknox_amd.workloads,
+    numpy seed 3) one after another -- oracle LDE + leaf rows, then the
+    reference's CPU Merkle forest -- `runs` times, median; every job's caps
+    compared with the committed expected outputs (tests/golden/workloads.json)."""
+    import hashlib
+    import statistics
+    import oracle_bind as ob
+    from zeknox_amd.workloads import mixed_batch_jobs
+    ref = ob.ref_lib()
+    with open(os.path.join(ROOT, "tests", "golden", "workloads.json")) as f:
+        exp = json.load(f)["batch64"]
+    jobs = mixed_batch_jobs()
+    inputs = [ob.salt(c << l, seed) for c, l, r, ht, seed in jobs]
+    ts, same = [], True
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        caps = []
+        for (c, l, r, ht, _), x in zip(jobs, inputs):
+            leaves = ob.transpose_rev(ob.lde(x, l, r, c), l + r, c)
+            caps.append(_ref_merkle(np, ob, ref, leaves, c, 1 << (l + r), ht)[1])
+        ts.append(time.perf_counter() - t0)
+        print(f"cpu config 3 run {len(ts)}: {ts[-1]:.1f} s", file=sys.stderr, flush=True)
+        same = same and all(hashlib.sha256(np.ascontiguousarray(cp, "<u8").tobytes()).hexdigest() == e["caps_sha256"]
+                            for cp, e in zip(caps, exp))
+    med = statistics.median(ts)
+    return {"value": round(64 / med, 3), "unit": "commits/s", "s_per_batch": round(med, 2),
+            "kind": "reference" if ref is not None else "port", "cores": ob.orc.orc_num_threads(),
+            "sample": f"{runs} runs of the 64 mixed jobs, sequential, median (oracle LDE + transpose, reference "
+                      "CPU Merkle)", "runs_s": [round(t, 2) for t in ts], "caps_equal_expected": same}
+
+
+def cpu_config4(zk, np, runs):
+    """Config 4 per job on the host cores: one 400 x 2^20, rate-1 Poseidon2
+    commit (oracle LDE + leaf rows, reference CPU Merkle), `runs` times,
+    median; caps compared with the same job through zkx_commit_batch."""
+    import statistics
+    import oracle_bind as ob
+    ref = ob.ref_lib()
+    cols, lg_n, rate, ht = 400, 20, 1, 3
+    x = ob.salt(cols << lg_n, 4000)
+    d = zk.DeviceBuffer.from_host(x)
+    c = zk.Commit(d, cols, lg_n, rate, CAP_H, ht, leaves=False)
+    zk.commit_batch([c])
+    gpu_caps = c.d_caps.to_host()
+    for b in (d, c.d_digests, c.d_caps):
+        b.free()
+    zk.lib.zkx_commit_trim(0)
+    ts, same = [], True
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        leaves = ob.transpose_rev(ob.lde(x, lg_n, rate, cols), lg_n + rate, cols)
+        _, caps = _ref_merkle(np, ob, ref, leaves, cols, 1 << (lg_n + rate), ht)
+        ts.append(time.perf_counter() - t0)
+        print(f"cpu config 4 run {len(ts)}: {ts[-1]:.1f} s", file=sys.stderr, flush=True)
+        same = same and bool(np.array_equal(caps, gpu_caps))
+        del leaves
+    med = statistics.median(ts)
+    return {"value": round(1.0 / med, 4), "unit": "commits/s", "s_per_commit": round(med, 2),
+            "kind": "reference" if ref is not None else "port", "cores": ob.orc.orc_num_threads(),
+            "sample": f"{runs} commits of one 400 x 2^20 rate-1 Poseidon2 job, median (oracle LDE + transpose, "
+                      "reference CPU Merkle)", "runs_s": [round(t, 2) for t in ts], "caps_equal_gpu": same}
+
+
+def issue_leg(zk, iters=8192):
+    """zkx_issue_probe: the achieved v_mad_u64_u32 rate (four waves per SIMD)
+    and the issue slots one i8 MFMA takes beside it."""
+    r = zk.issue_probe(iters)
+    nominal = r["simds"] * 2.4e9 / 4  # one wave-instruction per 4 cycles per SIMD at 2.4 GHz
+    return {"valu_lane_ops_per_s": r["valu_wave_insts_per_s"] * 64,
+            "valu_frac_of_nominal": round(r["valu_wave_insts_per_s"] / nominal, 3),
+            "mfma_issue_slots": round(r["mfma_issue_slots"], 3),
+            "mfma_alone_wave_insts_per_s": r["mfma_wave_insts_per_s"],
+            "ms_valu_mfma_mixed": [round(r["ms_valu"], 4), round(r["ms_mfma"], 4), round(r["ms_mixed"], 4)],
+            "mads_per_mfma": r["mads_per_mfma"], "waves_per_simd": r["waves_per_simd"]}
+
+
+def served_sweep(zk, np, rates, seconds, budget, streams):
+    """The served stream at several arrival rates: per rate commits/s, latency
+    percentiles and batches per request; the knee is the highest rate at
+    which the server keeps up (the last request completes within 5% + 0.5 s
+    of the stream's end) with a queueing p99 under one second."""
+    pts = []
+    for i, rate in enumerate(rates):
+        r = served_stream(zk, np, seconds, rate, budget, streams, seed=50 + i)
+        pts.append({"rate_hz": rate, "commits_per_s": r["commits_per_s"], "p50_ms": r["p50_ms"],
+                    "p99_ms": r["p99_ms"], "queue_p99_ms": r["queue_p99_ms"], "small_p99_ms": r["small_p99_ms"],
+                    "large_p99_ms": r["large_p99_ms"], "requests": r["requests"],
+                    "batches_per_request": round(r["batches"] / max(r["requests"], 1), 3),
+                    "max_batch": r["max_batch"], "peak_hbm_gb": r["peak_hbm_gb"], "elapsed_s": r["elapsed_s"]})
+        print(f"sweep {rate}/s: {r['commits_per_s']} commits/s, p50 {r['p50_ms']} ms, p99 {r['p99_ms']} ms",
+              file=sys.stderr, flush=True)
+    ok = [p for p in pts if p["elapsed_s"] <= 1.05 * seconds + 0.5 and (p["queue_p99_ms"] or 0) < 1000.0]
+    return {"workload": f"served stream (as below) at each rate for {seconds} s", "points": pts,
+            "knee_rate_hz": max((p["rate_hz"] for p in ok), default=None)}
 
 
 def _timed(zk, fn, steps, warmup):
@@ -195,7 +401,7 @@ def plain_vs_batched(zk, np, shapes, steps, warmup, seed, tune):
     return {"jobs": n, "plain_commits_per_s": round(n / plain_s, 2), "batched_commits_per_s": round(n / batch_s, 2),
             "batched_over_plain": round(plain_s / batch_s, 3), "batched_equals_plain": bool(same),
             "ms_per_batch": round(batch_s * 1e3, 2), "autotune_s": round(tune_s, 2),
-            "schedules (ntt_strided_max, ntt_even, merkle_tile_log)": [list(t) for t in schedules]}
+            "schedules (ntt_strided_max, ntt_even, merkle_tile_log, merkle_valu)": [list(t) for t in schedules]}
 
 
 def merkle_hashers(zk, np, steps, lg=20, cols=135, h=4):
@@ -295,21 +501,12 @@ def served_stream(zk, np, seconds, rate_hz, budget, streams, seed=5, trace_path=
     height 4, leaf rows kept.  Returns throughput and latency percentiles
     (latency = completion - submission, steady clock inside the server)."""
     import threading
-    rng = np.random.default_rng(seed)
+    from zeknox_amd.workloads import served_trace
     # one device pool of coefficients serves every request (its first cols*2^lg_n words)
     pool = zk.DeviceBuffer(400 << 20)
     pool.fill_random(seed)
     srv = zk.Server(budget, streams=streams, max_batch=64)
-    arrivals, t = [], 0.0
-    while True:
-        t += rng.exponential(1.0 / rate_hz)
-        if t >= seconds:
-            break
-        lg_n = int(np.clip(np.rint(rng.lognormal(np.log(14.0), 0.15)), 12, 20))
-        cols = int(rng.integers(2, 401))
-        rate = int(rng.integers(1, 4))
-        ht = 0 if rng.random() < 0.5 else 3
-        arrivals.append((t, cols, lg_n, rate, ht))
+    arrivals = served_trace(seed, seconds, rate_hz)
     ids, shape_of = [], {}
     done = threading.Event()
     lat, qwait, large, t_last, rows = [], [], [], [None], []
@@ -363,6 +560,7 @@ def served_stream(zk, np, seconds, rate_hz, budget, streams, seed=5, trace_path=
     return {"workload": f"Poisson {rate_hz}/s for {seconds} s; log2 degree lognormal(ln 14, 0.15) clipped [12,20], "
                         "cols U[2,400], rate_bits {1,2,3}, Poseidon/Poseidon2 50/50, cap_height 4",
             "requests": len(arrivals), "commits_per_s": round(len(arrivals) / elapsed, 2),
+            "elapsed_s": round(elapsed, 2),
             "p50_ms": pct(lat_ms, 50), "p99_ms": pct(lat_ms, 99), "mean_ms": round(float(lat_ms.mean()), 2),
             "queue_p50_ms": pct(q_ms, 50), "queue_p99_ms": pct(q_ms, 99),
             "small_p50_ms": pct(lat_ms[~large], 50), "small_p99_ms": pct(lat_ms[~large], 99),
@@ -431,13 +629,18 @@ def main():
     kernels = {k: {"launches_per_commit": n / args.steps, "ms_per_commit": ms / args.steps,
                    "avg_launch_ms": ms / n} for k, (n, ms) in stats.items()}
 
-    # ---- batched path: JOBS such commits per zkx_commit_batch call
+    # ---- batched path: JOBS such commits per zkx_commit_batch call (the
+    # engine captures a repeated batch's launches as a graph on its third
+    # call: replay is used only when the warm-up steps cover that capture)
+    graphs = not args.no_graphs and args.warmup >= 3
+    zk.set_commit_graphs(graphs)
     commits = [zk.Commit(d_in, COLS, LG_N, RATE, CAP_H, HASH) for _ in range(args.jobs)]
     if args.schedule:
         zk.set_tuning(zk.Tuning(*[int(v) for v in args.schedule.split(",")]), commits[0])
     elif not args.no_tune:  # one-time schedule search for the shape, outside the timing
         zk.autotune(commits[:1])
-    schedule = dict(zip(("ntt_strided_max", "ntt_even", "merkle_tile_log"), zk.get_tuning(commits[0]).astuple()))
+    schedule = dict(zip(("ntt_strided_max", "ntt_even", "merkle_tile_log", "merkle_valu"),
+                        zk.get_tuning(commits[0]).astuple()))
     for _ in range(args.warmup):
         zk.commit_batch(commits)
     zk.lib.zkx_device_synchronize(0)
@@ -462,30 +665,44 @@ def main():
                      and np.array_equal(pl[2], bl[2]))
     mode = "batched"
 
+    # ---- issue rates of this GPU (v_mad_u64_u32 alone; an i8 MFMA beside it)
+    issue = None if args.no_issue else issue_leg(zk)
+
     # ---- roofline of the dominant kernel: the leaf-hash launch of the batch
     # (Poseidon permutations, issue-bound: 64-bit modular arithmetic on the
     # 32-bit integer VALU, the MDS layers as i8 MFMAs that share the SIMD's
     # issue).  achieved = permutations per launch x issue slots per
-    # permutation (VALU wave-instructions + MFMA_ISSUE_SLOTS per MFMA) x 64
+    # permutation (VALU wave-instructions + the measured slots per MFMA) x 64
     # lanes / the launch's HIP-event duration (events on the library's
     # stream); peak = VALU_PEAK_TOPS (one wave-instruction per 4 cycles).
     dom = max(batch_kernels, key=lambda k: batch_kernels[k]["ms_per_step"])
     leaf = batch_kernels.get("merkle_leaves")
     roofline = None
-    if leaf:
-        tl = schedule["merkle_tile_log"]
+    if leaf and (schedule["merkle_tile_log"], schedule["merkle_valu"]) in VALU_WAVE_INSTS_PER_PERM:
+        tl = (schedule["merkle_tile_log"], schedule["merkle_valu"])
         perms = args.jobs * perms_per_commit(N, COLS, nc)
-        slots = VALU_WAVE_INSTS_PER_PERM[tl] + MFMA_ISSUE_SLOTS * MFMA_WAVE_INSTS_PER_PERM[tl]
-        achieved = perms * slots * 64 / (leaf["avg_launch_ms"] * 1e-3) / 1e12
+        t_launch = leaf["avg_launch_ms"] * 1e-3
+        mslots = issue["mfma_issue_slots"] if issue else MFMA_ISSUE_SLOTS
+        valu = perms * VALU_WAVE_INSTS_PER_PERM[tl] * 64 / t_launch / 1e12
+        mfma = perms * mslots * MFMA_WAVE_INSTS_PER_PERM[tl] * 64 / t_launch / 1e12
+        achieved = valu + mfma
         alg_bytes = args.jobs * (8 * COLS * N * 2 + 32 * nd)  # LDE read, leaf rows + digests written
         traffic = LEAF_TRAFFIC_BYTES_PER_COMMIT.get(tl)
+        digest = leaf_kernel_digest()
         roofline = {"bound": "valu+mfma issue", "achieved": round(achieved, 2), "peak": round(VALU_PEAK_TOPS, 2),
                     "unit": "T lane-issue-slots/s (32-bit VALU)", "frac": round(achieved / VALU_PEAK_TOPS, 3),
+                    "valu_frac": round(valu / VALU_PEAK_TOPS, 3), "mfma_slot_frac": round(mfma / VALU_PEAK_TOPS, 3),
+                    "frac_of_measured_mad_rate": round(achieved * 1e12 / issue["valu_lane_ops_per_s"], 3)
+                    if issue else None,
                     "traffic": round(traffic * args.jobs) if traffic else None, "algorithmic_bytes": alg_bytes,
-                    "hbm_gbs": round(traffic * args.jobs / (leaf["avg_launch_ms"] * 1e-3) / 1e9, 1) if traffic else None,
+                    "traffic_over_algorithmic": round(traffic * args.jobs / alg_bytes, 3) if traffic else None,
+                    "hbm_gbs": round(traffic * args.jobs / t_launch / 1e9, 1) if traffic else None,
+                    "hbm_frac": round(traffic * args.jobs / t_launch / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
                     "kernel": "merkle_leaves", "avg_launch_ms": round(leaf["avg_launch_ms"], 4),
                     "perms_per_launch": perms, "valu_wave_insts_per_perm": VALU_WAVE_INSTS_PER_PERM[tl],
-                    "mfma_wave_insts_per_perm": MFMA_WAVE_INSTS_PER_PERM[tl], "mfma_issue_slots": MFMA_ISSUE_SLOTS}
+                    "mfma_wave_insts_per_perm": MFMA_WAVE_INSTS_PER_PERM[tl], "mfma_issue_slots": mslots,
+                    "mfma_issue_slots_source": "measured (zkx_issue_probe)" if issue else "constant",
+                    "leaf_kernel_sources": digest, "counters_match_build": digest == LEAF_COUNTERS_SOURCES}
 
     # LDE passes against HBM: algorithmic bytes of the LDE (read n, write N
     # per column) over the batch's NTT launches
@@ -499,6 +716,20 @@ def main():
                         "traffic_gbs": round(NTT_TRAFFIC_BYTES_PER_COMMIT * args.jobs / (ntt["ms_per_step"] * 1e-3) / 1e9, 1),
                         "kernel": "batch_ntt (all passes)",
                         "note": "VALU-bound (64-bit modular butterflies), not HBM-bound"}
+        split = tuple(zk.ntt_pass_sizes(LG_N, zk.get_tuning(commits[0])))
+        ntt_valu = NTT_VALU_WAVE_INSTS_PER_COMMIT.get(split)
+        ntt_roofline["pass_split"] = list(split)
+        if ntt_valu:
+            # the bound that holds: VALU issue of the same launches
+            valu = ntt_valu * args.jobs * 64 / (ntt["ms_per_step"] * 1e-3) / 1e12
+            digest = leaf_kernel_digest(NTT_KERNEL_SOURCES)
+            ntt_roofline["valu_issue"] = {"achieved": round(valu, 2), "peak": round(VALU_PEAK_TOPS, 2),
+                                          "unit": "T lane-ops/s (32-bit VALU)", "frac": round(valu / VALU_PEAK_TOPS, 3),
+                                          "frac_of_measured_mad_rate": round(valu * 1e12 / issue["valu_lane_ops_per_s"], 3)
+                                          if issue else None,
+                                          "valu_wave_insts_per_commit": ntt_valu,
+                                          "ntt_kernel_sources": digest,
+                                          "counters_match_build": digest == NTT_COUNTERS_SOURCES}
 
     # ---- heterogeneous 64-job batch and the large Poseidon2 batch, both ways
     extra = {}
@@ -512,15 +743,31 @@ def main():
             extra["merkle_hashers"] = merkle_hashers(zk, np, args.extra_steps)
         if "ntt" in legs:
             extra["ntt"] = ntt_legs(zk, np, args.extra_steps)
+    else:
+        legs = set()
 
-    # ---- CPU baseline: whole commits of the same input on the host cores
+    # ---- CPU baseline: whole commits of the same input on the host cores;
+    # config 1 (the reference's CPU Merkle forest) beside it, and on request
+    # the 64-job batch and one 400 x 2^20 Poseidon2 job
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(np, coeffs, args.cpu_runs, d_cap.to_host())
+        c1 = cpu_config1(np, max(args.cpu_runs, 5))
+        c1_caps, c1_leaves = c1.pop("caps"), c1.pop("leaves")
+        cpu["configs"] = {"1": c1}
+        if "config1" in legs:
+            extra["config1"] = gpu_config1(zk, np, 4 * args.extra_steps, c1_caps, c1_leaves)
+        if "cpu3" in legs:
+            cpu["configs"]["3"] = cpu_config3(np, args.cpu_runs)
+        if "cpu4" in legs:
+            cpu["configs"]["4"] = cpu_config4(zk, np, args.cpu_runs)
+    if "sweep" in legs:
+        extra["served_sweep"] = served_sweep(zk, np, [float(r) for r in args.sweep_rates.split(",")],
+                                             args.sweep_seconds, 64 << 30, args.serve_streams)
 
     served = None
     if args.serve_seconds > 0:
-        served = served_stream(zk, np, args.serve_seconds, args.serve_rate, 64 << 30, 3,
+        served = served_stream(zk, np, args.serve_seconds, args.serve_rate, 64 << 30, args.serve_streams,
                                trace_path=args.serve_trace)
 
     value = args.jobs / batch_step_s
@@ -540,6 +787,7 @@ def main():
         "config": {"workload": "LDE+commit: 135 polys x 2^15, rate_bits 3 (2^18-point coset LDE, shift 7), "
                                "transpose to 2^18 leaves x 135, Poseidon Merkle, cap_height 4",
                    "path": mode, "commits_per_step": args.jobs, "parallelism": "single GPU",
+                   "graphs": graphs,
                    "schedule": schedule},
         "plain_commits_per_s": round(1.0 / plain_s, 2),
         "batched_over_plain": round(plain_s / batched_s, 3),
@@ -548,6 +796,7 @@ def main():
         "kernels_batched": batch_kernels,
         "dominant_kernel": dom,
         "roofline": roofline,
+        "issue_rates": issue,
         "ntt_roofline": ntt_roofline,
         "cpu_baseline": cpu,
         "served": served,

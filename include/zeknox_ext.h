@@ -70,6 +70,25 @@ RustError zkx_fill_digests_buf_linear_gpu_noalloc(void *digests_buf, void *cap_b
                                                   uint64_t leaves_buf_size, uint64_t leaf_size, uint64_t cap_height,
                                                   uint64_t hash_type, uint64_t gpu_id);
 
+/* Issue-rate probe of the device's SIMDs (the measured inputs of the hash
+ * kernels' roofline), four waves per SIMD on every SIMD, each kernel timed
+ * with HIP events over `iters` trips (a positive multiple of 4):
+ * full-rate 32-bit multiply-adds (v_mad_u64_u32, independent chains), i8
+ * matrix-core instructions alone (v_mfma_i32_32x32x32_i8, four independent
+ * accumulators), and both together (mads_per_mfma multiply-adds per MFMA);
+ * each the fastest of five launches after a warm-up launch.
+ * mfma_issue_slots = multiply-add issue slots one MFMA takes from its SIMD
+ * beside them: (t_mixed - t_valu) per MFMA over t_valu per multiply-add.
+ * No counterpart in the reference. */
+typedef struct {
+    double valu_wave_insts_per_s;  /* whole device */
+    double mfma_wave_insts_per_s;  /* whole device, MFMAs alone */
+    double mfma_issue_slots;
+    double ms_valu, ms_mfma, ms_mixed;
+    uint32_t simds, waves_per_simd, mads_per_mfma, reserved;
+} ZkxIssueRates;
+RustError zkx_issue_probe(size_t device_id, uint32_t iters, ZkxIssueRates *out);
+
 /* The stream the library enqueues its work on (a hipStream_t). */
 void *zkx_stream(size_t device_id);
 
@@ -103,15 +122,17 @@ RustError zkx_commit_trim(size_t device_id);
 /* ---- schedule autotuning -------------------------------------------------
  * How the commit of a shape (columns, log degree, rate bits, hash type, leaf
  * rows kept) is cut into launches: the LDE's DIF stages per strided pass
- * (1..9; greedy or spread evenly) and the leaves per Merkle work-group
- * (2^8..2^10).  No setting changes a result.  Shapes without an entry use
+ * (1..9; greedy or spread evenly), the leaves per Merkle work-group
+ * (2^8..2^10) and, for Poseidon leaves, whether the permutations' MDS layers
+ * run on the matrix cores or every step on the VALU.  No setting changes a
+ * result.  Shapes without an entry use
  * the default, which the C-ABI entry points (compute_batched_lde,
  * fill_digests_buf_linear_gpu_with_gpu_ptr) use as well. */
 typedef struct {
     uint32_t ntt_strided_max;  /* 1..9 */
     uint32_t ntt_even;         /* 0 greedy, 1 even */
     uint32_t merkle_tile_log;  /* 8..10 */
-    uint32_t reserved;
+    uint32_t merkle_valu;      /* Poseidon leaves: 0 MDS on the i8 matrix cores, 1 all-VALU */
 } ZkxTuning;
 
 /* Times every distinct shape among the jobs under each schedule (`iters`
@@ -124,6 +145,14 @@ RustError zkx_set_tuning(size_t device_id, const ZkxCommitJob *job, const ZkxTun
 /* on: zkx_commit_batch and the server tune each shape they have not seen
  * on its first commit (that commit takes longer). */
 RustError zkx_set_autotune(size_t device_id, int on);
+/* on (the default): zkx_commit_batch and the server's engines replay the
+ * launches of a batch whose job shapes and schedules they have run before
+ * as a captured HIP graph (one graph launch per batch; the descriptors are
+ * re-staged each call); off: every batch is launched kernel by kernel.  No
+ * setting changes a result. */
+RustError zkx_set_commit_graphs(size_t device_id, int on);
+/* Graphs the device's zkx_commit_batch engine holds (tests, benchmarks). */
+size_t zkx_commit_graphs_captured(size_t device_id);
 /* Stage counts of the DIF passes over 2^lg points under `tuning` (NULL: the
  * built-in default), top stages first; returns the pass count (at most
  * `cap` written), -1 for parameters out of range.  Host-only. */

@@ -126,9 +126,12 @@ def test_wait_any_returns_each_request_once(zk):
 
 
 def test_small_requests_run_beside_a_large_one(zk):
-    """A large request (LDE above 2^28 elements) runs alone on one engine;
-    small requests submitted after it start on the other engine before it
-    ends, and every result equals the same job run alone."""
+    """A large request (LDE above 2^28 elements) runs alone on one engine, its
+    launches on 7/8 of the device's work-group slots; small requests
+    submitted after it run on the other engine and finish while it runs
+    (each one's batch completes before the next starts, so every start lies
+    before the large request's end), and every result equals the same job
+    run alone."""
     big = (135, 19, 2, 4, 0)  # 135 x 2^21 LDE elements
     c, l, r, h, ht = big
     assert (c << (l + r)) > (1 << 28)

@@ -16,7 +16,8 @@ SHAPES = [  # (cols, lg_n, rate, hash)
     (3, 10, 3, 0),
     (40, 13, 3, 3),
 ]
-SCHEDULES = [(s, e, t) for (s, e), t in itertools.product([(9, 0), (9, 1), (6, 0), (6, 1), (4, 1)], [8, 9, 10])]
+SCHEDULES = [(s, e, t, v) for (s, e), t, v in itertools.product([(9, 0), (9, 1), (6, 0), (6, 1), (4, 1)], [8, 9, 10],
+                                                                 [0, 1]) if v == 0 or s == 9]
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -74,7 +75,7 @@ def test_autotune_records_a_valid_schedule(zk, tmp_path):
     c = zk.Commit(coeffs, cols, lg_n, rate, 4, ht)
     zk.autotune([c], iters=2)
     t = zk.get_tuning(c).astuple()
-    assert 1 <= t[0] <= 9 and t[1] in (0, 1) and 8 <= t[2] <= 10
+    assert 1 <= t[0] <= 9 and t[1] in (0, 1) and 8 <= t[2] <= 10 and t[3] in (0, 1)
     # the outputs of the tuning runs are the job's result
     want = oracle_commit(coeffs, cols, lg_n, rate, 4, ht)
     for a, b in zip(c.results(), want):
@@ -83,7 +84,7 @@ def test_autotune_records_a_valid_schedule(zk, tmp_path):
     path = tmp_path / "tune.txt"
     zk.tuning_save(str(path))
     zk.tuning_clear()
-    assert zk.get_tuning(c).astuple() == (9, 0, 8)
+    assert zk.get_tuning(c).astuple() == (9, 0, 8, 0)
     zk.tuning_load(str(path))
     assert zk.get_tuning(c).astuple() == t
     zk.tuning_clear()
@@ -113,3 +114,5 @@ def test_bad_schedule_rejected(zk):
         zk.set_tuning(zk.Tuning(9, 0, 11))
     with pytest.raises(zk.ZeknoxError):
         zk.set_tuning(zk.Tuning(0, 0, 8))
+    with pytest.raises(zk.ZeknoxError):
+        zk.set_tuning(zk.Tuning(9, 0, 8, 2))

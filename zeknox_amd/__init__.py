@@ -90,13 +90,21 @@ class CommitJob(ctypes.Structure):
 class Tuning(ctypes.Structure):
     """zeknox_ext.h ZkxTuning: schedule parameters of a commit shape."""
     _fields_ = [("ntt_strided_max", ctypes.c_uint32), ("ntt_even", ctypes.c_uint32),
-                ("merkle_tile_log", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("merkle_tile_log", ctypes.c_uint32), ("merkle_valu", ctypes.c_uint32)]
 
-    def __init__(self, ntt_strided_max=9, ntt_even=0, merkle_tile_log=8):
-        super().__init__(ntt_strided_max, ntt_even, merkle_tile_log, 0)
+    def __init__(self, ntt_strided_max=9, ntt_even=0, merkle_tile_log=8, merkle_valu=0):
+        super().__init__(ntt_strided_max, ntt_even, merkle_tile_log, merkle_valu)
 
     def astuple(self):
-        return (self.ntt_strided_max, self.ntt_even, self.merkle_tile_log)
+        return (self.ntt_strided_max, self.ntt_even, self.merkle_tile_log, self.merkle_valu)
+
+
+class IssueRates(ctypes.Structure):
+    """zeknox_ext.h ZkxIssueRates."""
+    _fields_ = [("valu_wave_insts_per_s", ctypes.c_double), ("mfma_wave_insts_per_s", ctypes.c_double),
+                ("mfma_issue_slots", ctypes.c_double), ("ms_valu", ctypes.c_double), ("ms_mfma", ctypes.c_double),
+                ("ms_mixed", ctypes.c_double), ("simds", ctypes.c_uint32), ("waves_per_simd", ctypes.c_uint32),
+                ("mads_per_mfma", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class ServeResult(ctypes.Structure):
@@ -148,6 +156,8 @@ def _load():
         "zkx_get_tuning": (RustError, [sz, vp, ctypes.POINTER(Tuning)]),
         "zkx_set_tuning": (RustError, [sz, vp, ctypes.POINTER(Tuning)]),
         "zkx_set_autotune": (RustError, [sz, ctypes.c_int]),
+        "zkx_set_commit_graphs": (RustError, [sz, ctypes.c_int]),
+        "zkx_commit_graphs_captured": (sz, [sz]),
         "zkx_tuning_clear": (RustError, [sz]),
         "zkx_tuning_save": (RustError, [sz, ctypes.c_char_p]),
         "zkx_tuning_load": (RustError, [sz, ctypes.c_char_p]),
@@ -157,6 +167,7 @@ def _load():
                                            ctypes.POINTER(u64)]),
         "zkx_compute_batched_lde_seeded": (RustError, [sz, vp, vp, u32, ctypes.c_int, NTTConfig, u64]),
         "zkx_device_synchronize": (RustError, [sz]),
+        "zkx_issue_probe": (RustError, [sz, u32, ctypes.POINTER(IssueRates)]),
         "zkx_stream": (vp, [sz]),
         "zkx_profile_enable": (RustError, [ctypes.c_int]),
         "zkx_kernel_stats": (ctypes.c_int, [vp, vp, vp, ctypes.c_int]),
@@ -444,6 +455,15 @@ def set_autotune(on, device_id=0):
     _check(lib.zkx_set_autotune(device_id, int(bool(on))))
 
 
+def set_commit_graphs(on, device_id=0):
+    """Replay repeated batch launch lists as captured HIP graphs (default on)."""
+    _check(lib.zkx_set_commit_graphs(device_id, 1 if on else 0))
+
+
+def commit_graphs_captured(device_id=0):
+    return lib.zkx_commit_graphs_captured(device_id)
+
+
 def tuning_clear(device_id=0):
     _check(lib.zkx_tuning_clear(device_id))
 
@@ -548,6 +568,14 @@ class Server:
 
 
 # ---- timing ---------------------------------------------------------------
+
+def issue_probe(iters=4096, device_id=0):
+    """zkx_issue_probe: the measured v_mad_u64_u32 rate and the issue slots
+    one i8 MFMA takes beside it, as a dict."""
+    r = IssueRates()
+    _check(lib.zkx_issue_probe(device_id, iters, ctypes.byref(r)))
+    return {f: getattr(r, f) for f, _ in IssueRates._fields_ if f != "reserved"}
+
 
 def profile_enable(on=True):
     _check(lib.zkx_profile_enable(1 if on else 0))

@@ -17,6 +17,7 @@
 // no bit-reversal or transpose pass; the leaf rows are written by the leaf
 // kernel as it reads them.
 #pragma once
+#include <map>
 #include <vector>
 
 #include "merkle.h"
@@ -54,8 +55,11 @@ public:
     // `forced`, when given, replaces every job's schedule parameters.
     // `lde_bufs`, when given, holds one working buffer per job instead
     // (job i: cols << (lg_n + rate) elements, 256-byte aligned).
+    // `eighths` < 8: the batch's persistent NTT passes and Merkle launches
+    // run on that share of the work-groups the device holds at once,
+    // leaving the rest to other streams (a large served request).
     void commit(const CommitJob *jobs, size_t n, void *scratch = nullptr, const Tuning *forced = nullptr,
-                uint64_t *const *lde_bufs = nullptr);
+                uint64_t *const *lde_bufs = nullptr, uint32_t eighths = 8);
     std::mutex mu;
     void sync();
     // After a failed commit or wait: the fused Merkle launches' arrival
@@ -67,8 +71,26 @@ public:
     hipStream_t stream() const { return stream_; }
     Device &device() { return dev_; }
 
+    // Graphs captured so far (tests and benchmarks).
+    size_t graphs_captured() const { return graphs_.size(); }
+    // Whether this engine may replay launch lists as graphs at all (the
+    // server's engines do not: their batches' compositions rarely repeat).
+    void set_graphs(bool on) { use_graphs_ = on; }
+
 private:
     void stage(const void *src, size_t bytes, size_t &offset);
+    // One launch of a batch: a descriptor list in desc_ and its grid.
+    struct Launch {
+        size_t offset;      // bytes into desc_
+        uint32_t count;
+        uint64_t blocks;
+        int kind;           // 0 NTT, 1 Merkle
+        int hash;           // NTT: contiguous; Merkle: hash type
+        int pipe;           // NTT: ntt_pipe_kind of the list
+        uint32_t tile_log;  // Merkle work-group size (log2 lanes)
+    };
+    void issue(hipStream_t s, const std::vector<Launch> &launches, uint32_t eighths);
+    void drop_graphs();
 
     Device &dev_;
     hipStream_t stream_ = nullptr;
@@ -80,6 +102,19 @@ private:
     void *pinned_ = nullptr;   // descriptor staging (host, pinned)
     size_t pinned_bytes_ = 0;
     hipEvent_t staged_ = nullptr;  // last staging copy
+    // A batch's launches depend on its job shapes and schedules and on the
+    // device addresses of the descriptor lists, never on the jobs' buffers
+    // (those are read from the staged descriptors).  A launch list issued a
+    // second time is captured as a HIP graph on cap_ (a stream that only
+    // records) and replayed on stream_ from then on.
+    struct GraphExec {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    bool use_graphs_ = true;
+    std::map<std::vector<uint64_t>, GraphExec> graphs_;
+    std::map<std::vector<uint64_t>, int> seen_;
+    hipStream_t cap_ = nullptr;
 };
 
 // The device's engine on the device stream (used by zkx_commit_batch).

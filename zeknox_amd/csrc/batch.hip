@@ -13,6 +13,10 @@ namespace zkx {
 namespace {
 
 constexpr size_t kAlign = 256;
+constexpr size_t kMaxGraphs = 64;  // captured launch lists kept per engine
+// A launch list is captured on its third sight: a capture and instantiation
+// costs host time worth many replays, so a list seen twice is not enough.
+constexpr int kCaptureAfter = 3;
 
 size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
 
@@ -54,6 +58,8 @@ Engine::~Engine()
 {
     (void)hipSetDevice(dev_.id);
     if (stream_) (void)hipStreamSynchronize(stream_);
+    drop_graphs();
+    if (cap_) (void)hipStreamDestroy(cap_);
     if (staged_) (void)hipEventDestroy(staged_);
     if (pinned_) (void)hipHostFree(pinned_);
     if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
@@ -68,8 +74,10 @@ void Engine::trim()
     scratch_ = DevBuf();
 }
 
-void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning *forced, uint64_t *const *lde_bufs)
+void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning *forced, uint64_t *const *lde_bufs,
+                    uint32_t eighths)
 {
+    if (eighths < 1 || eighths > 8) invalid("commit: device share out of range");
     if (n == 0) return;
     for (size_t i = 0; i < n; i++) check_commit_job(jobs[i]);
     if (forced && !valid_tuning(*forced)) invalid("schedule parameters out of range");
@@ -159,6 +167,7 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
         s.hash_type = j.hash_type;
         s.leaf_stride = 1;
         s.col_stride = 1ull << lg;
+        s.leaf_valu = tun[i].merkle_valu;
         check_merkle(s);
         std::vector<MerkleArgs> passes;
         plan_merkle(s, passes, tun[i].merkle_tile_log);
@@ -169,15 +178,6 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
 
     // ---- descriptor lists: NTT pass k of every job; every tree of each hash
     // kind and work-group size.  All staged with one host->device copy.
-    struct Launch {
-        size_t offset;  // bytes into desc_
-        uint32_t count;
-        uint64_t blocks;
-        int kind;       // 0 NTT, 1 Merkle
-        int hash;       // NTT: contiguous; Merkle: hash type
-        int pipe;       // NTT: ntt_pipe_kind of the list
-        uint32_t tile_log;  // Merkle work-group size (log2 lanes)
-    };
     std::vector<Launch> launches;
     std::vector<char> host;
     auto push = [&](const void *p, size_t bytes) {
@@ -239,28 +239,82 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
             host.resize(align_up(host.size()));
         }
     size_t off0 = 0;
+    const void *desc_before = desc_.p;
     stage(host.data(), host.size(), off0);
     (void)off0;
+    if (desc_.p != desc_before) drop_graphs();  // captured launches read the old lists
 
     // ---- launches, in dependency order on one stream (running the second
     // half of a batch's NTT passes beside the first half's hashing on a
     // second stream was measured 7% slower: the co-resident NTT work-groups
-    // take the hash kernel's occupancy)
+    // take the hash kernel's occupancy).  Per-kernel timing (profiling())
+    // needs events around each launch, so it runs the launches directly.
     try {
-        for (const Launch &l : launches) {
-            const char *base = static_cast<const char *>(desc_.p) + l.offset;
-            if (l.kind == 0) {
-                launch_ntt_list(stream_, reinterpret_cast<const NttPass *>(base), l.count, l.blocks, l.hash == 1,
-                                l.pipe, "batch_ntt");
-            } else {
-                launch_merkle_pass(stream_, l.hash, reinterpret_cast<const MerkleArgs *>(base), l.count,
-                                   MerkleArgs{}, l.blocks, l.tile_log);
+        if (use_graphs_ && dev_.commit_graphs && !profiling() && !forced) {  // (the autotuner times direct launches)
+            std::vector<uint64_t> key;
+            key.reserve(2 + 7 * launches.size());
+            key.push_back((uint64_t)(uintptr_t)desc_.p);
+            key.push_back(eighths);
+            for (const Launch &l : launches)
+                key.insert(key.end(), {(uint64_t)l.offset, l.count, l.blocks, (uint64_t)l.kind, (uint64_t)l.hash,
+                                       (uint64_t)l.pipe, l.tile_log});
+            auto it = graphs_.find(key);
+            if (it == graphs_.end() && seen_.size() >= 64 * kMaxGraphs) seen_.clear();
+            if (it == graphs_.end() && ++seen_[key] >= kCaptureAfter) {
+                if (graphs_.size() >= kMaxGraphs) drop_graphs();
+                if (!cap_) ZKX_HIP(hipStreamCreate(&cap_));
+                GraphExec ge;
+                ZKX_HIP(hipStreamBeginCapture(cap_, hipStreamCaptureModeThreadLocal));
+                try {
+                    issue(cap_, launches, eighths);
+                } catch (...) {
+                    hipGraph_t g = nullptr;
+                    (void)hipStreamEndCapture(cap_, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                ZKX_HIP(hipStreamEndCapture(cap_, &ge.graph));
+                const hipError_t e = hipGraphInstantiate(&ge.exec, ge.graph, nullptr, nullptr, 0);
+                if (e != hipSuccess) {
+                    (void)hipGraphDestroy(ge.graph);
+                    throw_hip(e, "hipGraphInstantiate", __FILE__, __LINE__);
+                }
+                it = graphs_.emplace(key, ge).first;
+            }
+            if (it != graphs_.end()) {
+                ZKX_HIP(hipGraphLaunch(it->second.exec, stream_));
+                return;
             }
         }
+        issue(stream_, launches, eighths);
     } catch (...) {
         ctr_dirty_ = true;
         throw;
     }
+}
+
+void Engine::issue(hipStream_t s, const std::vector<Launch> &launches, uint32_t eighths)
+{
+    for (const Launch &l : launches) {
+        const char *base = static_cast<const char *>(desc_.p) + l.offset;
+        if (l.kind == 0) {
+            launch_ntt_list(s, reinterpret_cast<const NttPass *>(base), l.count, l.blocks, l.hash == 1, l.pipe,
+                            "batch_ntt", eighths);
+        } else {
+            launch_merkle_pass(s, l.hash, reinterpret_cast<const MerkleArgs *>(base), l.count, MerkleArgs{},
+                               l.blocks, l.tile_log, eighths);
+        }
+    }
+}
+
+void Engine::drop_graphs()
+{
+    for (auto &kv : graphs_) {
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+        if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+    }
+    graphs_.clear();
+    seen_.clear();
 }
 
 Engine &default_engine(Device &d)

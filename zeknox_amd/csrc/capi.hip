@@ -385,7 +385,7 @@ RustError zkx_get_tuning(size_t device_id, const ZkxCommitJob *job, ZkxTuning *o
         Device &d = device(device_id);
         Tuning t = d.tuning_default();
         if (job) t = d.tuning_for(tune_key(to_jobs(job, 1)[0]));
-        *out = ZkxTuning{t.ntt_strided_max, t.ntt_even, t.merkle_tile_log, 0};
+        *out = ZkxTuning{t.ntt_strided_max, t.ntt_even, t.merkle_tile_log, t.merkle_valu};
     });
 }
 
@@ -397,6 +397,7 @@ RustError zkx_set_tuning(size_t device_id, const ZkxCommitJob *job, const ZkxTun
         v.ntt_strided_max = t->ntt_strided_max;
         v.ntt_even = t->ntt_even;
         v.merkle_tile_log = t->merkle_tile_log;
+        v.merkle_valu = t->merkle_valu;
         if (!valid_tuning(v)) invalid("schedule parameters out of range");
         Device &d = device(device_id);
         std::lock_guard<std::mutex> g(d.tune_mu);
@@ -413,6 +414,7 @@ int zkx_ntt_pass_sizes(uint32_t lg, const ZkxTuning *t, uint32_t *sizes, int cap
             v.ntt_strided_max = t->ntt_strided_max;
             v.ntt_even = t->ntt_even;
             v.merkle_tile_log = t->merkle_tile_log;
+            v.merkle_valu = t->merkle_valu;
         }
         if (!valid_tuning(v) || lg > 32) return -1;
         const std::vector<uint32_t> p = ntt_pass_sizes(lg, v);
@@ -453,6 +455,22 @@ int zkx_merkle_plan(uint64_t nl, uint64_t nc, uint32_t tile_log, uint64_t *count
     }
 }
 
+RustError zkx_set_commit_graphs(size_t device_id, int on)
+{
+    return guarded([&] { device(device_id).commit_graphs = on != 0; });
+}
+
+size_t zkx_commit_graphs_captured(size_t device_id)
+{
+    try {
+        Engine &e = default_engine(device(device_id));
+        std::lock_guard<std::mutex> g(e.mu);
+        return e.graphs_captured();
+    } catch (...) {
+        return 0;
+    }
+}
+
 RustError zkx_set_autotune(size_t device_id, int on)
 {
     return guarded([&] {
@@ -480,12 +498,13 @@ RustError zkx_tuning_save(size_t device_id, const char *path)
         std::lock_guard<std::mutex> g(d.tune_mu);
         FILE *f = fopen(path, "w");
         if (!f) invalid(std::string("cannot open ") + path);
-        fprintf(f, "# cols lg_n rate hash leaves : ntt_strided_max ntt_even merkle_tile_log\n");
+        fprintf(f, "# cols lg_n rate hash leaves : ntt_strided_max ntt_even merkle_tile_log merkle_valu\n");
         for (const auto &kv : d.tuned) {
             const TuneKey &k = kv.first;
             const Tuning &t = kv.second;
-            fprintf(f, "%u %u %u %u %d : %u %u %u\n", std::get<0>(k), std::get<1>(k), std::get<2>(k),
-                    std::get<3>(k), (int)std::get<4>(k), t.ntt_strided_max, t.ntt_even, t.merkle_tile_log);
+            fprintf(f, "%u %u %u %u %d : %u %u %u %u\n", std::get<0>(k), std::get<1>(k), std::get<2>(k),
+                    std::get<3>(k), (int)std::get<4>(k), t.ntt_strided_max, t.ntt_even, t.merkle_tile_log,
+                    t.merkle_valu);
         }
         const bool ok = fclose(f) == 0;
         if (!ok) invalid(std::string("cannot write ") + path);
@@ -506,9 +525,11 @@ RustError zkx_tuning_load(size_t device_id, const char *path)
         while (fgets(line, sizeof line, f)) {
             lineno++;
             if (line[0] == '#' || line[0] == '\n') continue;
-            unsigned c, n, r, h, s, e, t;
+            unsigned c, n, r, h, s, e, t, mv = 0;
             int lv;
-            if (sscanf(line, "%u %u %u %u %d : %u %u %u", &c, &n, &r, &h, &lv, &s, &e, &t) != 8) {
+            // the merkle_valu column is optional (tables written before it existed)
+            const int got = sscanf(line, "%u %u %u %u %d : %u %u %u %u", &c, &n, &r, &h, &lv, &s, &e, &t, &mv);
+            if (got != 8 && got != 9) {
                 bad = true;
                 break;
             }
@@ -516,6 +537,7 @@ RustError zkx_tuning_load(size_t device_id, const char *path)
             v.ntt_strided_max = s;
             v.ntt_even = e;
             v.merkle_tile_log = t;
+            v.merkle_valu = mv;
             if (!valid_tuning(v)) {
                 bad = true;
                 break;

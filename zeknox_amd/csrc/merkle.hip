@@ -4,6 +4,8 @@
 #include "merkle.h"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <string>
 
 #include "merkle_launch.h"
@@ -45,6 +47,7 @@ uint32_t plan_merkle(const MerkleSpec &s, MerkleArgs (&passes)[kMerkleMaxPasses]
     a.leaf_size = (uint32_t)s.leaf_size;
     a.leaf_stride = s.leaf_stride;
     a.col_stride = s.col_stride;
+    a.leaf_valu = s.leaf_valu;
     if (s.num_caps == s.num_leaves) {
         a.count = s.num_leaves;
         a.lg_w = 0;
@@ -97,8 +100,32 @@ uint64_t fuse_merkle_passes(const std::vector<MerkleArgs> &passes, MerkleArgs &o
     return fuse_merkle_passes(passes.data(), (uint32_t)passes.size(), out);
 }
 
+namespace {
+
+template <int HT>
+void launch_hash(hipStream_t stream, const MerkleArgs *dev_list, uint32_t n, const MerkleArgs &one, uint64_t blocks,
+                 uint32_t tile_log_max, uint32_t eighths)
+{
+    uint64_t groups = 0;  // one work-group per tile
+    if (eighths < 8) {
+        // resident work-groups per (device, tile size), cached
+        static std::mutex mu;
+        static std::map<std::pair<int, uint32_t>, uint64_t> cache;
+        int dev = 0;
+        ZKX_HIP(hipGetDevice(&dev));
+        std::lock_guard<std::mutex> g(mu);
+        auto it = cache.find({dev, tile_log_max});
+        if (it == cache.end()) it = cache.emplace(std::make_pair(dev, tile_log_max),
+                                                  merkle_resident_groups<HT>(tile_log_max)).first;
+        if (it->second) groups = std::max<uint64_t>(it->second * eighths / 8, 1);
+    }
+    launch_merkle_hash<HT>(stream, dev_list, n, one, blocks, tile_log_max, groups);
+}
+
+}  // namespace
+
 void launch_merkle_pass(hipStream_t stream, int hash_type, const MerkleArgs *dev_list,
-                        uint32_t n, const MerkleArgs &one, uint64_t blocks, uint32_t tile_log_max)
+                        uint32_t n, const MerkleArgs &one, uint64_t blocks, uint32_t tile_log_max, uint32_t eighths)
 {
     if (blocks == 0) return;
     if (blocks > 0x7FFFFFFFull) invalid("tree too large for one launch");
@@ -106,13 +133,11 @@ void launch_merkle_pass(hipStream_t stream, int hash_type, const MerkleArgs *dev
         invalid("Merkle tile size out of range");
     ProfScope ps("merkle_leaves", stream);
     switch (hash_type) {
-    case kPoseidon2: launch_merkle_hash<kPoseidon2>(stream, dev_list, n, one, blocks, tile_log_max); break;
-    case kKeccak: launch_merkle_hash<kKeccak>(stream, dev_list, n, one, blocks, tile_log_max); break;
-    case kMonolith: launch_merkle_hash<kMonolith>(stream, dev_list, n, one, blocks, tile_log_max); break;
-    case kPoseidonBN128:
-        launch_merkle_hash<kPoseidonBN128>(stream, dev_list, n, one, blocks, tile_log_max);
-        break;
-    default: launch_merkle_hash<kPoseidon>(stream, dev_list, n, one, blocks, tile_log_max); break;
+    case kPoseidon2: launch_hash<kPoseidon2>(stream, dev_list, n, one, blocks, tile_log_max, eighths); break;
+    case kKeccak: launch_hash<kKeccak>(stream, dev_list, n, one, blocks, tile_log_max, eighths); break;
+    case kMonolith: launch_hash<kMonolith>(stream, dev_list, n, one, blocks, tile_log_max, eighths); break;
+    case kPoseidonBN128: launch_hash<kPoseidonBN128>(stream, dev_list, n, one, blocks, tile_log_max, eighths); break;
+    default: launch_hash<kPoseidon>(stream, dev_list, n, one, blocks, tile_log_max, eighths); break;
     }
     ZKX_HIP(hipGetLastError());
 }
@@ -120,9 +145,12 @@ void launch_merkle_pass(hipStream_t stream, int hash_type, const MerkleArgs *dev
 void merkle_build(Device &d, const MerkleSpec &s)
 {
     check_merkle(s);
-    const uint32_t tl = d.tuning_default().merkle_tile_log;
+    const Tuning tun = d.tuning_default();
+    const uint32_t tl = tun.merkle_tile_log;
+    MerkleSpec sv = s;
+    sv.leaf_valu = tun.merkle_valu;
     MerkleArgs passes[kMerkleMaxPasses];
-    const uint32_t np = plan_merkle(s, passes, tl);
+    const uint32_t np = plan_merkle(sv, passes, tl);
     MerkleArgs f;
     const uint64_t nctr = fuse_merkle_passes(passes, np, f);
     f.tile_begin = 0;

@@ -34,6 +34,8 @@ struct MerkleArgs {
     uint32_t next_tile_log[kMerkleMaxPasses - 1];
     uint32_t next_ctr[kMerkleMaxPasses - 1];  // first arrival counter of each upper pass
     uint32_t *ctr;           // arrival counters, zero before and after the launch (null: n_next == 0)
+    uint32_t leaf_valu;      // Poseidon leaves: 1 = all-VALU permutations (no matrix-core MDS)
+    uint32_t pad_;
 };
 
 }  // namespace zkx

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <map>
 #include <tuple>
 #include <mutex>
@@ -104,6 +105,9 @@ struct Device {
     Tuning tune_default;
     std::map<TuneKey, Tuning> tuned;
     bool autotune_new_shapes = false;  // tune unseen shapes in zkx_commit_batch
+    // Batch engines replay launch lists they have issued before as HIP
+    // graphs (zkx_set_commit_graphs); read at each commit.
+    std::atomic<bool> commit_graphs{true};
     Tuning tuning_default();
     // The shape's entry, or the default (found = false).
     Tuning tuning_for(const TuneKey &k, bool *found = nullptr);

@@ -98,7 +98,7 @@ MerkleArgs *Device::merkle_descriptor()
 bool valid_tuning(const Tuning &t)
 {
     return t.ntt_strided_max >= 1 && t.ntt_strided_max <= 9 && t.ntt_even <= 1 && t.merkle_tile_log >= 8 &&
-           t.merkle_tile_log <= 10;
+           t.merkle_tile_log <= 10 && t.merkle_valu <= 1;
 }
 
 Tuning Device::tuning_default()

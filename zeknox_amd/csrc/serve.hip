@@ -54,6 +54,7 @@ Server::Server(Device &d, size_t budget_bytes, int nstreams, size_t max_batch_jo
     place_.reset(budget_);
     for (int i = 0; i < nstreams; i++) {
         engines_.push_back(std::make_unique<Engine>(d));
+        engines_.back()->set_graphs(false);  // batch compositions rarely repeat
         slots_.push_back(std::make_unique<Slot>());
         ZKX_HIP(hipEventCreateWithFlags(&slots_.back()->done, hipEventDisableTiming));
     }
@@ -182,18 +183,19 @@ void Server::assign_locked(size_t e, std::vector<ServeRequest *> batch, bool lar
 void Server::admit_locked()
 {
     if (stop_) return;
-    auto free_slot = [&]() -> long {
-        for (size_t e = 0; e < slots_.size(); e++)
+    // the first free engine from `first` on (small batches try engine 0 first)
+    auto free_slot = [&](size_t first) -> long {
+        for (size_t e = first; e < slots_.size(); e++)
             if (!slots_[e]->busy) return (long)e;
         return -1;
     };
     // large requests: arrival order, one per batch, placed from the top of
-    // the region, leaving one engine to small batches (when there is more
-    // than one)
-    const int lanes = slots_.size() > 1 ? (int)slots_.size() - 1 : 1;
-    while (!large_q_.empty() && large_running_ < lanes) {
+    // the region, on engines 1.. (engine 0 stays with small batches when
+    // there is more than one engine)
+    const size_t large_first = slots_.size() > 1 ? 1 : 0;
+    while (!large_q_.empty()) {
         ServeRequest *r = large_q_.front();
-        const long e = free_slot();
+        const long e = free_slot(large_first);
         if (e < 0 || !place_.take_high(r->bytes, r->off)) break;
         large_q_.pop_front();
         used_ += r->bytes;
@@ -206,7 +208,7 @@ void Server::admit_locked()
     const ServeRequest *blocked = large_q_.empty() ? nullptr : large_q_.front();
     const size_t limit = blocked ? budget_ - blocked->bytes : budget_;
     while (!small_q_.empty()) {
-        const long e = free_slot();
+        const long e = free_slot(0);
         if (e < 0) break;
         std::vector<ServeRequest *> batch;
         uint64_t w = 0;
@@ -246,7 +248,7 @@ bool Server::launch(size_t e, std::vector<ServeRequest *> &batch, std::string &e
         {
             std::lock_guard<std::mutex> g(eng.mu);
             try {
-                eng.commit(jobs.data(), jobs.size(), nullptr, nullptr, lde.data());
+                eng.commit(jobs.data(), jobs.size(), nullptr, nullptr, lde.data(), slot.large ? kLargeShare : 8);
             } catch (...) {
                 eng.mark_failed();
                 throw;

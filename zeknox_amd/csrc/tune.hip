@@ -2,8 +2,9 @@
 //
 // The search space is what changes the work decomposition without changing
 // a result: how the DIF stages of the LDE are cut into passes (strided passes
-// of at most 4..9 stages, greedy or evenly spread) and how many leaves one
-// Merkle work-group hashes and reduces in LDS (2^8..2^10).  Candidates that
+// of at most 4..9 stages, greedy or evenly spread), how many leaves one
+// Merkle work-group hashes and reduces in LDS (2^8..2^10) and, for Poseidon
+// leaves, whether the MDS layers run on the matrix cores or on the VALU.  Candidates that
 // give the same launch plan are timed once.  Times are HIP events on the
 // engine's stream around whole commits, fastest of `iters`.
 //
@@ -95,6 +96,17 @@ Tuning autotune(Engine &e, const CommitJob &job, int iters)
         if (tl == ntt_best.merkle_tile_log || (tl > lg_leaves && ntt_best.merkle_tile_log >= lg_leaves)) continue;
         Tuning c = ntt_best;
         c.merkle_tile_log = tl;
+        const double ms = time_commit(e, batch, c, iters, tm);
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+    }
+
+    // Poseidon leaves: MDS layers on the matrix cores or all-VALU permutations
+    if (job.hash_type == kPoseidon && job.cols > 4) {
+        Tuning c = best;
+        c.merkle_valu = 1 - best.merkle_valu;
         const double ms = time_commit(e, batch, c, iters, tm);
         if (ms < best_ms) {
             best_ms = ms;

@@ -18,7 +18,9 @@ struct Tuning {
     uint32_t ntt_even = 0;
     // Merkle: log2 lanes (leaves) per work-group tile, 8..10.
     uint32_t merkle_tile_log = 8;
-    uint32_t reserved = 0;
+    // Merkle, Poseidon leaves: 0 = the MDS layers of whole-wave leaf
+    // permutations on the i8 matrix cores; 1 = every permutation on the VALU.
+    uint32_t merkle_valu = 0;
 };
 
 // A commit shape: (columns, log2 degree, rate bits, hash type, leaf rows kept).
