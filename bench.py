@@ -100,7 +100,9 @@ def parse():
                     "(no captured graph replay)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="served-stream duration (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=40.0, help="served-stream arrivals per second")
+    ap.add_argument("--serve-rate", type=float, default=200.0,
+                    help="served-stream arrivals per second (default: about 80% of the highest rate the server "
+                         "keeps up with, 240-260/s in the sweeps)")
     ap.add_argument("--serve-streams", type=int, default=3, help="commit engines (streams) of the server")
     ap.add_argument("--serve-trace", default=None,
                     help="write one CSV line per served request (times in ms from the first arrival)")

@@ -55,11 +55,8 @@ public:
     // `forced`, when given, replaces every job's schedule parameters.
     // `lde_bufs`, when given, holds one working buffer per job instead
     // (job i: cols << (lg_n + rate) elements, 256-byte aligned).
-    // `eighths` < 8: the batch's persistent NTT passes and Merkle launches
-    // run on that share of the work-groups the device holds at once,
-    // leaving the rest to other streams (a large served request).
     void commit(const CommitJob *jobs, size_t n, void *scratch = nullptr, const Tuning *forced = nullptr,
-                uint64_t *const *lde_bufs = nullptr, uint32_t eighths = 8);
+                uint64_t *const *lde_bufs = nullptr);
     std::mutex mu;
     void sync();
     // After a failed commit or wait: the fused Merkle launches' arrival
@@ -89,7 +86,7 @@ private:
         int pipe;           // NTT: ntt_pipe_kind of the list
         uint32_t tile_log;  // Merkle work-group size (log2 lanes)
     };
-    void issue(hipStream_t s, const std::vector<Launch> &launches, uint32_t eighths);
+    void issue(hipStream_t s, const std::vector<Launch> &launches);
     void drop_graphs();
 
     Device &dev_;
@@ -105,8 +102,8 @@ private:
     // A batch's launches depend on its job shapes and schedules and on the
     // device addresses of the descriptor lists, never on the jobs' buffers
     // (those are read from the staged descriptors).  A launch list issued a
-    // second time is captured as a HIP graph on cap_ (a stream that only
-    // records) and replayed on stream_ from then on.
+    // third time is captured as a HIP graph (on a short-lived stream that
+    // only records) and replayed on stream_ from then on.
     struct GraphExec {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
@@ -114,7 +111,6 @@ private:
     bool use_graphs_ = true;
     std::map<std::vector<uint64_t>, GraphExec> graphs_;
     std::map<std::vector<uint64_t>, int> seen_;
-    hipStream_t cap_ = nullptr;
 };
 
 // The device's engine on the device stream (used by zkx_commit_batch).

@@ -59,7 +59,6 @@ Engine::~Engine()
     (void)hipSetDevice(dev_.id);
     if (stream_) (void)hipStreamSynchronize(stream_);
     drop_graphs();
-    if (cap_) (void)hipStreamDestroy(cap_);
     if (staged_) (void)hipEventDestroy(staged_);
     if (pinned_) (void)hipHostFree(pinned_);
     if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
@@ -74,10 +73,8 @@ void Engine::trim()
     scratch_ = DevBuf();
 }
 
-void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning *forced, uint64_t *const *lde_bufs,
-                    uint32_t eighths)
+void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning *forced, uint64_t *const *lde_bufs)
 {
-    if (eighths < 1 || eighths > 8) invalid("commit: device share out of range");
     if (n == 0) return;
     for (size_t i = 0; i < n; i++) check_commit_job(jobs[i]);
     if (forced && !valid_tuning(*forced)) invalid("schedule parameters out of range");
@@ -252,9 +249,8 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
     try {
         if (use_graphs_ && dev_.commit_graphs && !profiling() && !forced) {  // (the autotuner times direct launches)
             std::vector<uint64_t> key;
-            key.reserve(2 + 7 * launches.size());
+            key.reserve(1 + 7 * launches.size());
             key.push_back((uint64_t)(uintptr_t)desc_.p);
-            key.push_back(eighths);
             for (const Launch &l : launches)
                 key.insert(key.end(), {(uint64_t)l.offset, l.count, l.blocks, (uint64_t)l.kind, (uint64_t)l.hash,
                                        (uint64_t)l.pipe, l.tile_log});
@@ -262,18 +258,26 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
             if (it == graphs_.end() && seen_.size() >= 64 * kMaxGraphs) seen_.clear();
             if (it == graphs_.end() && ++seen_[key] >= kCaptureAfter) {
                 if (graphs_.size() >= kMaxGraphs) drop_graphs();
-                if (!cap_) ZKX_HIP(hipStreamCreate(&cap_));
+                // captured on a stream of its own that only records and is
+                // destroyed right after: no stream is kept for it
+                hipStream_t cap = nullptr;
+                ZKX_HIP(hipStreamCreate(&cap));
                 GraphExec ge;
-                ZKX_HIP(hipStreamBeginCapture(cap_, hipStreamCaptureModeThreadLocal));
-                try {
-                    issue(cap_, launches, eighths);
-                } catch (...) {
-                    hipGraph_t g = nullptr;
-                    (void)hipStreamEndCapture(cap_, &g);
-                    if (g) (void)hipGraphDestroy(g);
-                    throw;
+                hipError_t ce = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+                if (ce == hipSuccess) {
+                    try {
+                        issue(cap, launches);
+                    } catch (...) {
+                        hipGraph_t g = nullptr;
+                        (void)hipStreamEndCapture(cap, &g);
+                        if (g) (void)hipGraphDestroy(g);
+                        (void)hipStreamDestroy(cap);
+                        throw;
+                    }
+                    ce = hipStreamEndCapture(cap, &ge.graph);
                 }
-                ZKX_HIP(hipStreamEndCapture(cap_, &ge.graph));
+                (void)hipStreamDestroy(cap);
+                if (ce != hipSuccess) throw_hip(ce, "stream capture of a batch", __FILE__, __LINE__);
                 const hipError_t e = hipGraphInstantiate(&ge.exec, ge.graph, nullptr, nullptr, 0);
                 if (e != hipSuccess) {
                     (void)hipGraphDestroy(ge.graph);
@@ -286,23 +290,23 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
                 return;
             }
         }
-        issue(stream_, launches, eighths);
+        issue(stream_, launches);
     } catch (...) {
         ctr_dirty_ = true;
         throw;
     }
 }
 
-void Engine::issue(hipStream_t s, const std::vector<Launch> &launches, uint32_t eighths)
+void Engine::issue(hipStream_t s, const std::vector<Launch> &launches)
 {
     for (const Launch &l : launches) {
         const char *base = static_cast<const char *>(desc_.p) + l.offset;
         if (l.kind == 0) {
             launch_ntt_list(s, reinterpret_cast<const NttPass *>(base), l.count, l.blocks, l.hash == 1, l.pipe,
-                            "batch_ntt", eighths);
+                            "batch_ntt");
         } else {
             launch_merkle_pass(s, l.hash, reinterpret_cast<const MerkleArgs *>(base), l.count, MerkleArgs{},
-                               l.blocks, l.tile_log, eighths);
+                               l.blocks, l.tile_log);
         }
     }
 }

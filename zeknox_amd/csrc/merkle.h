@@ -34,11 +34,8 @@ uint64_t fuse_merkle_passes(const MerkleArgs *passes, uint32_t n, MerkleArgs &ou
 // One launch over `n` job descriptors in device memory (n > 0), or over the
 // single descriptor `one` (n == 0); `blocks` work-groups of 2^tile_log_max
 // lanes in total (every descriptor's tile_log <= tile_log_max).
-// With eighths < 8 the launch runs on that share of the work-groups the
-// device holds at once (each looping over its tiles).
 void launch_merkle_pass(hipStream_t stream, int hash_type, const MerkleArgs *dev_list,
-                        uint32_t n, const MerkleArgs &one, uint64_t blocks, uint32_t tile_log_max,
-                        uint32_t eighths = 8);
+                        uint32_t n, const MerkleArgs &one, uint64_t blocks, uint32_t tile_log_max);
 // The whole tree in one launch on the device stream; call with d.mu held.
 void merkle_build(Device &d, const MerkleSpec &s);
 

@@ -4,8 +4,6 @@
 #include "merkle.h"
 
 #include <algorithm>
-#include <map>
-#include <mutex>
 #include <string>
 
 #include "merkle_launch.h"
@@ -100,32 +98,8 @@ uint64_t fuse_merkle_passes(const std::vector<MerkleArgs> &passes, MerkleArgs &o
     return fuse_merkle_passes(passes.data(), (uint32_t)passes.size(), out);
 }
 
-namespace {
-
-template <int HT>
-void launch_hash(hipStream_t stream, const MerkleArgs *dev_list, uint32_t n, const MerkleArgs &one, uint64_t blocks,
-                 uint32_t tile_log_max, uint32_t eighths)
-{
-    uint64_t groups = 0;  // one work-group per tile
-    if (eighths < 8) {
-        // resident work-groups per (device, tile size), cached
-        static std::mutex mu;
-        static std::map<std::pair<int, uint32_t>, uint64_t> cache;
-        int dev = 0;
-        ZKX_HIP(hipGetDevice(&dev));
-        std::lock_guard<std::mutex> g(mu);
-        auto it = cache.find({dev, tile_log_max});
-        if (it == cache.end()) it = cache.emplace(std::make_pair(dev, tile_log_max),
-                                                  merkle_resident_groups<HT>(tile_log_max)).first;
-        if (it->second) groups = std::max<uint64_t>(it->second * eighths / 8, 1);
-    }
-    launch_merkle_hash<HT>(stream, dev_list, n, one, blocks, tile_log_max, groups);
-}
-
-}  // namespace
-
 void launch_merkle_pass(hipStream_t stream, int hash_type, const MerkleArgs *dev_list,
-                        uint32_t n, const MerkleArgs &one, uint64_t blocks, uint32_t tile_log_max, uint32_t eighths)
+                        uint32_t n, const MerkleArgs &one, uint64_t blocks, uint32_t tile_log_max)
 {
     if (blocks == 0) return;
     if (blocks > 0x7FFFFFFFull) invalid("tree too large for one launch");
@@ -133,11 +107,13 @@ void launch_merkle_pass(hipStream_t stream, int hash_type, const MerkleArgs *dev
         invalid("Merkle tile size out of range");
     ProfScope ps("merkle_leaves", stream);
     switch (hash_type) {
-    case kPoseidon2: launch_hash<kPoseidon2>(stream, dev_list, n, one, blocks, tile_log_max, eighths); break;
-    case kKeccak: launch_hash<kKeccak>(stream, dev_list, n, one, blocks, tile_log_max, eighths); break;
-    case kMonolith: launch_hash<kMonolith>(stream, dev_list, n, one, blocks, tile_log_max, eighths); break;
-    case kPoseidonBN128: launch_hash<kPoseidonBN128>(stream, dev_list, n, one, blocks, tile_log_max, eighths); break;
-    default: launch_hash<kPoseidon>(stream, dev_list, n, one, blocks, tile_log_max, eighths); break;
+    case kPoseidon2: launch_merkle_hash<kPoseidon2>(stream, dev_list, n, one, blocks, tile_log_max); break;
+    case kKeccak: launch_merkle_hash<kKeccak>(stream, dev_list, n, one, blocks, tile_log_max); break;
+    case kMonolith: launch_merkle_hash<kMonolith>(stream, dev_list, n, one, blocks, tile_log_max); break;
+    case kPoseidonBN128:
+        launch_merkle_hash<kPoseidonBN128>(stream, dev_list, n, one, blocks, tile_log_max);
+        break;
+    default: launch_merkle_hash<kPoseidon>(stream, dev_list, n, one, blocks, tile_log_max); break;
     }
     ZKX_HIP(hipGetLastError());
 }

@@ -57,12 +57,12 @@ __device__ __forceinline__ gptr<uint64_t> node_ptr(const MerkleArgs &a, uint64_t
     return gmem(a.dig) + 4 * (sub * a.sdl + ((1ull << lg_w) - 2) + idx);
 }
 
-__device__ __forceinline__ const MerkleArgs *find_job(const MerkleArgs *jobs, uint32_t njobs, uint64_t b)
+__device__ __forceinline__ const MerkleArgs *find_job(const MerkleArgs *jobs, uint32_t njobs)
 {
     uint32_t lo = 0, hi = njobs;
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (jobs[mid].tile_begin <= b) lo = mid;
+        if (jobs[mid].tile_begin <= blockIdx.x) lo = mid;
         else hi = mid;
     }
     return jobs + lo;
@@ -125,18 +125,20 @@ __device__ __forceinline__ void leaf_hash_mfma(const MerkleArgs &a, uint64_t fir
     __syncthreads();
 }
 
-// Tile b of the launch (every return is uniform across the work-group).
 template <int HT, int TPB>
-__device__ __forceinline__ void merkle_tile(const MerkleArgs *__restrict__ jobs, uint32_t njobs, const MerkleArgs &one,
-                                            uint64_t b, uint64_t (*lds)[4], uint32_t &lds_last, uint64_t *kt)
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(merkle_waves_per_eu(HT))))
+k_merkle_tile(const MerkleArgs *__restrict__ jobs, uint32_t njobs, MerkleArgs one)
 {
-    const MerkleArgs *job = njobs ? find_job(jobs, njobs, b) : nullptr;
+    __shared__ uint64_t lds[TPB][4];
+    __shared__ uint32_t lds_last;
+    __shared__ uint64_t kt[HT == kPoseidon ? kMfmaRcWords : 1];
+    const MerkleArgs *job = njobs ? find_job(jobs, njobs) : nullptr;
     const MerkleArgs a = njobs ? *job : one;
     uint32_t up = 0;  // upper passes done
     const uint32_t t = threadIdx.x;
     uint32_t cur = 1u << a.tile_log;  // nodes of this tile at the current level
     uint32_t lg_w = a.lg_w;           // log2 per-subtree width of the current level
-    uint64_t first = (b - a.tile_begin) << a.tile_log;  // first node of the tile
+    uint64_t first = ((uint64_t)blockIdx.x - a.tile_begin) << a.tile_log;  // first node of the tile
 
     uint64_t h[4];
     const bool mfma = HT == kPoseidon && cur >= 64 && a.leaf_size > 4 && !a.leaf_valu;
@@ -251,35 +253,6 @@ __device__ __forceinline__ void merkle_tile(const MerkleArgs *__restrict__ jobs,
         }
         cur = half;
         __syncthreads();
-    }
-}
-
-// One work-group per tile: work-group b runs tile b.
-template <int HT, int TPB>
-__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(merkle_waves_per_eu(HT))))
-k_merkle_tile(const MerkleArgs *__restrict__ jobs, uint32_t njobs, MerkleArgs one)
-{
-    __shared__ uint64_t lds[TPB][4];
-    __shared__ uint32_t lds_last;
-    __shared__ uint64_t kt[HT == kPoseidon ? kMfmaRcWords : 1];
-    merkle_tile<HT, TPB>(jobs, njobs, one, blockIdx.x, lds, lds_last, kt);
-}
-
-// A grid smaller than the tile count (a descriptor list, njobs > 0):
-// work-group g runs tiles g, g + gridDim.x, ... below ntiles, leaving the
-// rest of the device's work-group slots to other streams' kernels (a large
-// served request's leaf launch beside small batches).  Nothing waits across
-// tiles, so any grid size completes.
-template <int HT, int TPB>
-__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(merkle_waves_per_eu(HT))))
-k_merkle_tiles_strided(const MerkleArgs *__restrict__ jobs, uint32_t njobs, uint64_t ntiles)
-{
-    __shared__ uint64_t lds[TPB][4];
-    __shared__ uint32_t lds_last;
-    __shared__ uint64_t kt[HT == kPoseidon ? kMfmaRcWords : 1];
-    for (uint64_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
-        merkle_tile<HT, TPB>(jobs, njobs, jobs[0], b, lds, lds_last, kt);
-        __syncthreads();  // the tile's LDS reads are done before the next tile writes
     }
 }
 

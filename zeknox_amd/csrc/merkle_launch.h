@@ -8,14 +8,10 @@
 
 namespace zkx {
 
-// One launch over `blocks` tiles of 2^tile_log_max lanes of the n
-// descriptors at dev_list (n > 0) or the single descriptor `one` (n == 0),
-// with min(blocks, max_groups) work-groups (max_groups 0: one per tile).
+// One launch of `blocks` work-groups of 2^tile_log_max lanes over the n
+// descriptors at dev_list (n > 0) or the single descriptor `one` (n == 0).
 template <int HT>
 void launch_merkle_hash(hipStream_t stream, const MerkleArgs *dev_list, uint32_t n, const MerkleArgs &one,
-                        uint64_t blocks, uint32_t tile_log_max, uint64_t max_groups);
-// Work-groups of that launch the device holds at once.
-template <int HT>
-uint64_t merkle_resident_groups(uint32_t tile_log_max);
+                        uint64_t blocks, uint32_t tile_log_max);
 
 }  // namespace zkx

@@ -49,10 +49,9 @@ int ntt_pipe_kind(const NttPass *host_list, size_t n);
 
 // One launch over n pass descriptors in device memory (sorted by
 // tile_begin), all contiguous (s_lo == 0) or all strided; `pipe_kind` as
-// ntt_pipe_kind of the same list.  A persistent (pipelined) kernel's grid
-// takes eighths/8 of the work-groups the device holds at once.
+// ntt_pipe_kind of the same list.
 void launch_ntt_list(hipStream_t stream, const NttPass *dev_list, uint32_t n, uint64_t blocks, bool contiguous,
-                     int pipe_kind, const char *name, uint32_t eighths = 8);
+                     int pipe_kind, const char *name);
 
 // table[i] = base^i, i < n.
 void fill_powers(uint64_t *dst, uint64_t n, uint64_t base, hipStream_t s);

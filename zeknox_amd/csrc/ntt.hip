@@ -36,40 +36,39 @@ uint64_t resident_groups(Kernel kernel)
     return n;
 }
 
-// eighths: the share of the resident work-groups the grid may use (8: all)
 template <class Kernel>
 void launch_persistent(Kernel kernel, hipStream_t stream, const NttPass *dev_list, uint32_t n, const NttPass &one,
-                       uint64_t tiles, uint32_t eighths = 8)
+                       uint64_t tiles)
 {
-    const uint64_t groups = std::min(tiles, std::max<uint64_t>(resident_groups(kernel) * eighths / 8, 1));
+    const uint64_t groups = std::min(tiles, resident_groups(kernel));
     hipLaunchKernelGGL(kernel, dim3((uint32_t)groups), dim3(kNttThreads), 0, stream, dev_list, n, one, tiles);
 }
 
 template <int MODE>
 void launch_strided_pipe(int m, hipStream_t stream, const NttPass *dev_list, uint32_t n, const NttPass &one,
-                         uint64_t tiles, uint32_t eighths)
+                         uint64_t tiles)
 {
     switch (m) {
-    case 1: launch_persistent(k_ntt_strided_pipe<MODE, 1>, stream, dev_list, n, one, tiles, eighths); break;
-    case 2: launch_persistent(k_ntt_strided_pipe<MODE, 2>, stream, dev_list, n, one, tiles, eighths); break;
-    case 3: launch_persistent(k_ntt_strided_pipe<MODE, 3>, stream, dev_list, n, one, tiles, eighths); break;
-    default: launch_persistent(k_ntt_strided_pipe<MODE, 4>, stream, dev_list, n, one, tiles, eighths); break;
+    case 1: launch_persistent(k_ntt_strided_pipe<MODE, 1>, stream, dev_list, n, one, tiles); break;
+    case 2: launch_persistent(k_ntt_strided_pipe<MODE, 2>, stream, dev_list, n, one, tiles); break;
+    case 3: launch_persistent(k_ntt_strided_pipe<MODE, 3>, stream, dev_list, n, one, tiles); break;
+    default: launch_persistent(k_ntt_strided_pipe<MODE, 4>, stream, dev_list, n, one, tiles); break;
     }
 }
 
 // A launch over the passes (device list, or `one` when n == 0) with the
 // kernel ntt_pipe_kind chose; false for kind 0.
 bool launch_pipe(int kind, hipStream_t stream, const NttPass *dev_list, uint32_t n, const NttPass &one,
-                 uint64_t tiles, uint32_t eighths = 8)
+                 uint64_t tiles)
 {
     if (kind == 1) {
-        launch_persistent(k_ntt_contig_pipe, stream, dev_list, n, one, tiles, eighths);
+        launch_persistent(k_ntt_contig_pipe, stream, dev_list, n, one, tiles);
         return true;
     }
     if (kind >= 2 && kind < 10) {
         const int m = (kind - 2) % 4 + 1;
-        if (kind >= 6) launch_strided_pipe<kSrcLde>(m, stream, dev_list, n, one, tiles, eighths);
-        else launch_strided_pipe<kSrcInPlace>(m, stream, dev_list, n, one, tiles, eighths);
+        if (kind >= 6) launch_strided_pipe<kSrcLde>(m, stream, dev_list, n, one, tiles);
+        else launch_strided_pipe<kSrcInPlace>(m, stream, dev_list, n, one, tiles);
         return true;
     }
     return false;
@@ -125,12 +124,12 @@ int ntt_pipe_kind(const NttPass *host_list, size_t n)
 }
 
 void launch_ntt_list(hipStream_t stream, const NttPass *dev_list, uint32_t n, uint64_t blocks, bool contiguous,
-                     int pipe_kind, const char *name, uint32_t eighths)
+                     int pipe_kind, const char *name)
 {
     if (blocks == 0) return;
     if (blocks > 0x7FFFFFFFull) invalid("transform batch too large for one launch");
     ProfScope ps(name, stream);
-    if (launch_pipe(pipe_kind, stream, dev_list, n, NttPass{}, blocks, eighths)) {
+    if (launch_pipe(pipe_kind, stream, dev_list, n, NttPass{}, blocks)) {
     }
     else if (contiguous)
         hipLaunchKernelGGL(k_ntt_tiles<true>, dim3((uint32_t)blocks), dim3(kNttThreads), 0, stream, dev_list, n,

@@ -12,11 +12,9 @@
 //     large one in its own batch;
 //   * large requests run one per batch, in arrival order among themselves,
 //     on engines 1..nstreams-1: engine 0 only ever takes small batches, so
-//     a small request is never queued behind large ones for an engine; and
-//     a large batch's persistent NTT passes and leaf launch take at most
-//     kLargeShare eighths of the work-groups the device holds (each
-//     work-group looping over tiles), so a small batch's work-groups find
-//     free slots instead of waiting for the large launch to drain;
+//     a small request is never queued behind large ones for an engine (on
+//     the device its work-groups still compete with those of a large
+//     request's launches);
 //   * the budget is one device allocation made when the server starts; a
 //     request is admitted only when its working set -- output buffers (leaf
 //     rows, digests, caps) followed by its LDE scratch -- can be placed in it
@@ -77,10 +75,6 @@ public:
     // also the work cap of a batch of small requests (about 20 ms of device
     // time at the batched rate).
     static constexpr uint64_t kLargeWork = 1ull << 28;
-    // A large request's persistent NTT passes and leaf launch run on this
-    // many eighths of the work-groups the device holds, so small batches on
-    // engine 0 always find free slots.
-    static constexpr uint32_t kLargeShare = 7;
 
     Server(Device &d, size_t budget_bytes, int nstreams, size_t max_batch_jobs);
     ~Server();
