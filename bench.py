@@ -47,19 +47,19 @@ HBM_PEAK_GBS = 8000.0
 #   by Merkle work-group size (log2 lanes).  The leaf permutations run their
 #   MDS layers on the matrix cores (v_mfma_i32_32x32x32_i8).
 #   keyed by (merkle_tile_log, merkle_valu)
-VALU_WAVE_INSTS_PER_PERM = {(8, 0): 236.80, (9, 0): 231.97, (10, 0): 229.57, (9, 1): 310.03}
+VALU_WAVE_INSTS_PER_PERM = {(8, 0): 236.78, (9, 0): 231.96, (10, 0): 229.52, (9, 1): 310.05}
 MFMA_WAVE_INSTS_PER_PERM = {(8, 0): 7.969, (9, 0): 7.969, (10, 0): 7.969, (9, 1): 0.0}
 #   SHA-256 (first 16 hex digits) of the leaf kernel's sources those counts
 #   were measured on (leaf_kernel_digest()); a build whose sources differ is
 #   flagged in the JSON ("counters_match_build": false)
-LEAF_COUNTERS_SOURCES = "6877227349fca0b4"
+LEAF_COUNTERS_SOURCES = "02983ad8b7021e36"
 #   issue slots (one per 4 cycles) that one such MFMA takes from the vector
 #   issue of its SIMD beside full-rate VALU work: measured live by the
 #   "issue" leg (zkx_issue_probe); this value only when that leg is skipped
 MFMA_ISSUE_SLOTS = 3.25
 #   HBM bytes per commit: FETCH_SIZE x 2 (the counter reads half the bytes of
 #   a plain 1 GiB copy) + WRITE_SIZE x 1, KiB x 1024, by work-group size
-LEAF_TRAFFIC_BYTES_PER_COMMIT = {(8, 0): 8.184e8, (9, 0): 8.222e8, (10, 0): 8.196e8, (9, 1): 8.629e8}
+LEAF_TRAFFIC_BYTES_PER_COMMIT = {(8, 0): 8.175e8, (9, 0): 8.089e8, (10, 0): 8.117e8, (9, 1): 8.509e8}
 # Sources of the leaf-hash launch (zeknox_amd/csrc)
 LEAF_KERNEL_SOURCES = ("gl64.h", "gl_constants.h", "gmem.h", "hash.h", "poseidon_mfma.h", "merkle_kernels.h",
                        "merkle_launch.h", "merkle_launch_impl.h", "merkle_types.h", "merkle_poseidon.hip")

@@ -127,7 +127,12 @@ RustError zkx_commit_trim(size_t device_id);
  * run on the matrix cores or every step on the VALU.  No setting changes a
  * result.  Shapes without an entry use
  * the default, which the C-ABI entry points (compute_batched_lde,
- * fill_digests_buf_linear_gpu_with_gpu_ptr) use as well. */
+ * fill_digests_buf_linear_gpu_with_gpu_ptr) use as well, except that until
+ * zkx_set_tuning sets a default the Merkle entry points size the work-groups
+ * of one tree by its leaf count (2^8 lanes up to 2^16 leaves, 2^(lg-8) at
+ * 2^17 and 2^18, then 2^9 for Poseidon and Poseidon2, 2^10 for Poseidon
+ * over BN254, 2^8 for the others: the fastest for a tree alone), where
+ * batches use the default's 2^8. */
 typedef struct {
     uint32_t ntt_strided_max;  /* 1..9 */
     uint32_t ntt_even;         /* 0 greedy, 1 even */

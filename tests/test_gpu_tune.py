@@ -69,6 +69,25 @@ def test_c_abi_path_under_a_non_default_schedule(zk):
         zk.tuning_clear()
 
 
+@pytest.mark.parametrize("sched", [None, (4, 1, 8, 0), (4, 1, 9, 1), (4, 1, 10, 0)])
+def test_c_abi_merkle_default_and_set_defaults(zk, sched):
+    """The C-ABI Merkle entry point hashes with 2^10-lane work-groups until a
+    default is set, then with the default's; zkx_tuning_clear restores the
+    built-in one.  Poseidon, 12-element leaves (matrix-core or VALU form)."""
+    cols, lg_l, cap_h = 12, 13, 4
+    leaves = ob.canonical(cols << lg_l, 4100)
+    want = ob.merkle(leaves, cols, cap_h, 0)
+    if sched is not None:
+        zk.set_tuning(zk.Tuning(*sched))
+    try:
+        for rep in range(2):
+            got = zk.merkle_tree(leaves, cols, cap_h, 0)
+            assert all(np.array_equal(g, w) for g, w in zip(got, want)), f"schedule {sched} call {rep}"
+    finally:
+        zk.tuning_clear()
+    assert zk.get_tuning().astuple() == zk.Tuning().astuple()
+
+
 def test_autotune_records_a_valid_schedule(zk, tmp_path):
     cols, lg_n, rate, ht = 30, 13, 2, 0
     coeffs = ob.canonical(cols << lg_n, 5)

@@ -55,8 +55,11 @@ public:
     // `forced`, when given, replaces every job's schedule parameters.
     // `lde_bufs`, when given, holds one working buffer per job instead
     // (job i: cols << (lg_n + rate) elements, 256-byte aligned).
+    // `merkle_chunk` > 0: every Merkle launch is issued in pieces of at most
+    // that many work-groups (a large served request, so that other streams'
+    // kernels are dispatched between the pieces).
     void commit(const CommitJob *jobs, size_t n, void *scratch = nullptr, const Tuning *forced = nullptr,
-                uint64_t *const *lde_bufs = nullptr);
+                uint64_t *const *lde_bufs = nullptr, uint64_t merkle_chunk = 0);
     std::mutex mu;
     void sync();
     // After a failed commit or wait: the fused Merkle launches' arrival
@@ -86,7 +89,7 @@ private:
         int pipe;           // NTT: ntt_pipe_kind of the list
         uint32_t tile_log;  // Merkle work-group size (log2 lanes)
     };
-    void issue(hipStream_t s, const std::vector<Launch> &launches);
+    void issue(hipStream_t s, const std::vector<Launch> &launches, uint64_t merkle_chunk);
     void drop_graphs();
 
     Device &dev_;

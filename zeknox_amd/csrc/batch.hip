@@ -73,7 +73,8 @@ void Engine::trim()
     scratch_ = DevBuf();
 }
 
-void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning *forced, uint64_t *const *lde_bufs)
+void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning *forced, uint64_t *const *lde_bufs,
+                    uint64_t merkle_chunk)
 {
     if (n == 0) return;
     for (size_t i = 0; i < n; i++) check_commit_job(jobs[i]);
@@ -249,8 +250,9 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
     try {
         if (use_graphs_ && dev_.commit_graphs && !profiling() && !forced) {  // (the autotuner times direct launches)
             std::vector<uint64_t> key;
-            key.reserve(1 + 7 * launches.size());
+            key.reserve(2 + 7 * launches.size());
             key.push_back((uint64_t)(uintptr_t)desc_.p);
+            key.push_back(merkle_chunk);
             for (const Launch &l : launches)
                 key.insert(key.end(), {(uint64_t)l.offset, l.count, l.blocks, (uint64_t)l.kind, (uint64_t)l.hash,
                                        (uint64_t)l.pipe, l.tile_log});
@@ -266,7 +268,7 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
                 hipError_t ce = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
                 if (ce == hipSuccess) {
                     try {
-                        issue(cap, launches);
+                        issue(cap, launches, merkle_chunk);
                     } catch (...) {
                         hipGraph_t g = nullptr;
                         (void)hipStreamEndCapture(cap, &g);
@@ -290,14 +292,14 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
                 return;
             }
         }
-        issue(stream_, launches);
+        issue(stream_, launches, merkle_chunk);
     } catch (...) {
         ctr_dirty_ = true;
         throw;
     }
 }
 
-void Engine::issue(hipStream_t s, const std::vector<Launch> &launches)
+void Engine::issue(hipStream_t s, const std::vector<Launch> &launches, uint64_t merkle_chunk)
 {
     for (const Launch &l : launches) {
         const char *base = static_cast<const char *>(desc_.p) + l.offset;
@@ -306,7 +308,7 @@ void Engine::issue(hipStream_t s, const std::vector<Launch> &launches)
                             "batch_ntt");
         } else {
             launch_merkle_pass(s, l.hash, reinterpret_cast<const MerkleArgs *>(base), l.count, MerkleArgs{},
-                               l.blocks, l.tile_log);
+                               l.blocks, l.tile_log, merkle_chunk);
         }
     }
 }

@@ -401,8 +401,12 @@ RustError zkx_set_tuning(size_t device_id, const ZkxCommitJob *job, const ZkxTun
         if (!valid_tuning(v)) invalid("schedule parameters out of range");
         Device &d = device(device_id);
         std::lock_guard<std::mutex> g(d.tune_mu);
-        if (job) d.tuned[tune_key(to_jobs(job, 1)[0])] = v;
-        else d.tune_default = v;
+        if (job) {
+            d.tuned[tune_key(to_jobs(job, 1)[0])] = v;
+        } else {
+            d.tune_default = v;
+            d.tune_default_set = true;
+        }
     });
 }
 
@@ -487,6 +491,7 @@ RustError zkx_tuning_clear(size_t device_id)
         std::lock_guard<std::mutex> g(d.tune_mu);
         d.tuned.clear();
         d.tune_default = Tuning{};
+        d.tune_default_set = false;
     });
 }
 

@@ -34,8 +34,12 @@ uint64_t fuse_merkle_passes(const MerkleArgs *passes, uint32_t n, MerkleArgs &ou
 // One launch over `n` job descriptors in device memory (n > 0), or over the
 // single descriptor `one` (n == 0); `blocks` work-groups of 2^tile_log_max
 // lanes in total (every descriptor's tile_log <= tile_log_max).
+// chunk > 0 (a descriptor list, n > 0): issued as launches of at most
+// `chunk` work-groups each, in order on the stream, so kernels of other
+// streams can be dispatched between them.
 void launch_merkle_pass(hipStream_t stream, int hash_type, const MerkleArgs *dev_list,
-                        uint32_t n, const MerkleArgs &one, uint64_t blocks, uint32_t tile_log_max);
+                        uint32_t n, const MerkleArgs &one, uint64_t blocks, uint32_t tile_log_max,
+                        uint64_t chunk = 0);
 // The whole tree in one launch on the device stream; call with d.mu held.
 void merkle_build(Device &d, const MerkleSpec &s);
 

@@ -99,29 +99,33 @@ uint64_t fuse_merkle_passes(const std::vector<MerkleArgs> &passes, MerkleArgs &o
 }
 
 void launch_merkle_pass(hipStream_t stream, int hash_type, const MerkleArgs *dev_list,
-                        uint32_t n, const MerkleArgs &one, uint64_t blocks, uint32_t tile_log_max)
+                        uint32_t n, const MerkleArgs &one, uint64_t blocks, uint32_t tile_log_max, uint64_t chunk)
 {
     if (blocks == 0) return;
     if (blocks > 0x7FFFFFFFull) invalid("tree too large for one launch");
     if (tile_log_max < (uint32_t)kMerkleMinTileLog || tile_log_max > (uint32_t)kMerkleMaxTileLog)
         invalid("Merkle tile size out of range");
     ProfScope ps("merkle_leaves", stream);
-    switch (hash_type) {
-    case kPoseidon2: launch_merkle_hash<kPoseidon2>(stream, dev_list, n, one, blocks, tile_log_max); break;
-    case kKeccak: launch_merkle_hash<kKeccak>(stream, dev_list, n, one, blocks, tile_log_max); break;
-    case kMonolith: launch_merkle_hash<kMonolith>(stream, dev_list, n, one, blocks, tile_log_max); break;
-    case kPoseidonBN128:
-        launch_merkle_hash<kPoseidonBN128>(stream, dev_list, n, one, blocks, tile_log_max);
-        break;
-    default: launch_merkle_hash<kPoseidon>(stream, dev_list, n, one, blocks, tile_log_max); break;
+    const uint64_t step = (chunk && n > 0) ? chunk : blocks;
+    for (uint64_t b0 = 0; b0 < blocks; b0 += step) {
+        const uint64_t nb = std::min(step, blocks - b0);
+        switch (hash_type) {
+        case kPoseidon2: launch_merkle_hash<kPoseidon2>(stream, dev_list, n, one, nb, tile_log_max, (uint32_t)b0); break;
+        case kKeccak: launch_merkle_hash<kKeccak>(stream, dev_list, n, one, nb, tile_log_max, (uint32_t)b0); break;
+        case kMonolith: launch_merkle_hash<kMonolith>(stream, dev_list, n, one, nb, tile_log_max, (uint32_t)b0); break;
+        case kPoseidonBN128:
+            launch_merkle_hash<kPoseidonBN128>(stream, dev_list, n, one, nb, tile_log_max, (uint32_t)b0);
+            break;
+        default: launch_merkle_hash<kPoseidon>(stream, dev_list, n, one, nb, tile_log_max, (uint32_t)b0); break;
+        }
+        ZKX_HIP(hipGetLastError());
     }
-    ZKX_HIP(hipGetLastError());
 }
 
 void merkle_build(Device &d, const MerkleSpec &s)
 {
     check_merkle(s);
-    const Tuning tun = d.tuning_default();
+    const Tuning tun = d.cabi_merkle_tuning(s.num_leaves, s.hash_type);
     const uint32_t tl = tun.merkle_tile_log;
     MerkleSpec sv = s;
     sv.leaf_valu = tun.merkle_valu;

@@ -33,7 +33,9 @@
 //
 // A launch covers a list of independent jobs (trees or tree passes);
 // work-group b runs tile b - tile_begin of the job with the largest
-// tile_begin <= b.
+// tile_begin <= b, b = blockIdx.x + block0: a list may be issued as several
+// launches over consecutive ranges of b (nothing waits across work-groups,
+// and the arrival counters work across launches).
 #pragma once
 #include "gmem.h"
 #include "hash.h"
@@ -57,12 +59,12 @@ __device__ __forceinline__ gptr<uint64_t> node_ptr(const MerkleArgs &a, uint64_t
     return gmem(a.dig) + 4 * (sub * a.sdl + ((1ull << lg_w) - 2) + idx);
 }
 
-__device__ __forceinline__ const MerkleArgs *find_job(const MerkleArgs *jobs, uint32_t njobs)
+__device__ __forceinline__ const MerkleArgs *find_job(const MerkleArgs *jobs, uint32_t njobs, uint32_t bid)
 {
     uint32_t lo = 0, hi = njobs;
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (jobs[mid].tile_begin <= blockIdx.x) lo = mid;
+        if (jobs[mid].tile_begin <= bid) lo = mid;
         else hi = mid;
     }
     return jobs + lo;
@@ -127,18 +129,19 @@ __device__ __forceinline__ void leaf_hash_mfma(const MerkleArgs &a, uint64_t fir
 
 template <int HT, int TPB>
 __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(merkle_waves_per_eu(HT))))
-k_merkle_tile(const MerkleArgs *__restrict__ jobs, uint32_t njobs, MerkleArgs one)
+k_merkle_tile(const MerkleArgs *__restrict__ jobs, uint32_t njobs, MerkleArgs one, uint32_t block0)
 {
     __shared__ uint64_t lds[TPB][4];
     __shared__ uint32_t lds_last;
     __shared__ uint64_t kt[HT == kPoseidon ? kMfmaRcWords : 1];
-    const MerkleArgs *job = njobs ? find_job(jobs, njobs) : nullptr;
+    const uint32_t bid = blockIdx.x + block0;  // this work-group's tile in the whole launch list
+    const MerkleArgs *job = njobs ? find_job(jobs, njobs, bid) : nullptr;
     const MerkleArgs a = njobs ? *job : one;
     uint32_t up = 0;  // upper passes done
     const uint32_t t = threadIdx.x;
     uint32_t cur = 1u << a.tile_log;  // nodes of this tile at the current level
     uint32_t lg_w = a.lg_w;           // log2 per-subtree width of the current level
-    uint64_t first = ((uint64_t)blockIdx.x - a.tile_begin) << a.tile_log;  // first node of the tile
+    uint64_t first = ((uint64_t)bid - a.tile_begin) << a.tile_log;  // first node of the tile
 
     uint64_t h[4];
     const bool mfma = HT == kPoseidon && cur >= 64 && a.leaf_size > 4 && !a.leaf_valu;

@@ -9,9 +9,10 @@
 namespace zkx {
 
 // One launch of `blocks` work-groups of 2^tile_log_max lanes over the n
-// descriptors at dev_list (n > 0) or the single descriptor `one` (n == 0).
+// descriptors at dev_list (n > 0) or the single descriptor `one` (n == 0),
+// running tiles block0 .. block0 + blocks - 1.
 template <int HT>
 void launch_merkle_hash(hipStream_t stream, const MerkleArgs *dev_list, uint32_t n, const MerkleArgs &one,
-                        uint64_t blocks, uint32_t tile_log_max);
+                        uint64_t blocks, uint32_t tile_log_max, uint32_t block0);
 
 }  // namespace zkx

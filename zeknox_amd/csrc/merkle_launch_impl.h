@@ -8,12 +8,14 @@ namespace zkx {
 
 template <int HT>
 void launch_merkle_hash(hipStream_t stream, const MerkleArgs *dev_list, uint32_t n, const MerkleArgs &one,
-                        uint64_t blocks, uint32_t tile_log_max)
+                        uint64_t blocks, uint32_t tile_log_max, uint32_t block0)
 {
     const dim3 g((uint32_t)blocks);
-    if (tile_log_max == 10) hipLaunchKernelGGL((k_merkle_tile<HT, 1024>), g, dim3(1024), 0, stream, dev_list, n, one);
-    else if (tile_log_max == 9) hipLaunchKernelGGL((k_merkle_tile<HT, 512>), g, dim3(512), 0, stream, dev_list, n, one);
-    else hipLaunchKernelGGL((k_merkle_tile<HT, 256>), g, dim3(256), 0, stream, dev_list, n, one);
+    if (tile_log_max == 10)
+        hipLaunchKernelGGL((k_merkle_tile<HT, 1024>), g, dim3(1024), 0, stream, dev_list, n, one, block0);
+    else if (tile_log_max == 9)
+        hipLaunchKernelGGL((k_merkle_tile<HT, 512>), g, dim3(512), 0, stream, dev_list, n, one, block0);
+    else hipLaunchKernelGGL((k_merkle_tile<HT, 256>), g, dim3(256), 0, stream, dev_list, n, one, block0);
 }
 
 }  // namespace zkx

@@ -78,7 +78,7 @@ void merkle_noalloc(void *dig, void *cap, void *leaves, uint64_t nd, uint64_t nc
     Device &d = device(gpu_id);
     std::lock_guard<std::mutex> g(d.mu);
     unsigned char *const base = scratch_of(d);
-    const Tuning tun = d.tuning_default();
+    const Tuning tun = d.cabi_merkle_tuning(s.num_leaves, s.hash_type);
     const uint32_t tl = tun.merkle_tile_log;
     s.leaf_valu = tun.merkle_valu;
     MerkleArgs passes[kMerkleMaxPasses];

@@ -115,34 +115,36 @@ __device__ __forceinline__ void mds_mfma(uint64_t s[2][6], const MfmaOps &o, con
             const uint64_t x0 = s[t][2 * kappa] ^ XM, x1 = s[t][2 * kappa + 1] ^ XM;
             b[kappa] = mf_v4i{(int)gl::lo32(x0), (int)gl::hi32(x0), (int)gl::lo32(x1), (int)gl::hi32(x1)};
         }
-        mf_v16i d[3];
-#pragma unroll
-        for (int beta = 0; beta < 3; beta++) {
-            d[beta] = __builtin_amdgcn_mfma_i32_32x32x32_i8(o.a[beta][0], b[0], mf_v16i{}, 0, 0, 0);
-            d[beta] = __builtin_amdgcn_mfma_i32_32x32x32_i8(o.a[beta][1], b[1], d[beta], 0, 0, 0);
-            d[beta] = __builtin_amdgcn_mfma_i32_32x32x32_i8(o.a[beta][2], b[2], d[beta], 0, 0, 0);
-        }
         const bool own = mfma_own(o.h, t);
         const uint32_t dl = own ? gl::lo32(s[t][0]) : 0u, dh = own ? gl::hi32(s[t][0]) : 0u;
         const uint64_t *k = krow + 12 * ((o.h + t) & 1);
+        // one M-block at a time, recombined as soon as its chain is done: 16
+        // accumulator VGPRs live instead of 48 (the operands b and dl, dh
+        // hold what the later blocks need of the old state)
 #pragma unroll
-        for (int e = 0; e < 6; e++) {
-            const mf_v16i &dd = d[e >> 1];
-            const int q = 8 * (e & 1);  // registers q..q+3: bytes 0-3, q+4..q+7: bytes 4-7
-            const int32_t p01 = dd[q] + (int32_t)((uint32_t)dd[q + 1] << 8);
-            const int32_t p23 = dd[q + 2] + (int32_t)((uint32_t)dd[q + 3] << 8);
-            const int32_t q01 = dd[q + 4] + (int32_t)((uint32_t)dd[q + 5] << 8);
-            const int32_t q23 = dd[q + 6] + (int32_t)((uint32_t)dd[q + 7] << 8);
-            const int64_t one = (int32_t)gl::opaque(1u), m16 = (int32_t)gl::opaque(65536u);
-            uint64_t L = gl::vopaque64((uint64_t)((int64_t)p01 * one + (int64_t)k[2 * e]));
-            uint64_t H = gl::vopaque64((uint64_t)((int64_t)q01 * one + (int64_t)k[2 * e + 1]));
-            if (e == 0) {
-                L = gl::vopaque64((uint64_t)dl * gl::opaque(8u) + L);
-                H = gl::vopaque64((uint64_t)dh * gl::opaque(8u) + H);
+        for (int beta = 0; beta < 3; beta++) {
+            mf_v16i dd = __builtin_amdgcn_mfma_i32_32x32x32_i8(o.a[beta][0], b[0], mf_v16i{}, 0, 0, 0);
+            dd = __builtin_amdgcn_mfma_i32_32x32x32_i8(o.a[beta][1], b[1], dd, 0, 0, 0);
+            dd = __builtin_amdgcn_mfma_i32_32x32x32_i8(o.a[beta][2], b[2], dd, 0, 0, 0);
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                const int e = 2 * beta + u;
+                const int q = 8 * u;  // registers q..q+3: bytes 0-3, q+4..q+7: bytes 4-7
+                const int32_t p01 = dd[q] + (int32_t)((uint32_t)dd[q + 1] << 8);
+                const int32_t p23 = dd[q + 2] + (int32_t)((uint32_t)dd[q + 3] << 8);
+                const int32_t q01 = dd[q + 4] + (int32_t)((uint32_t)dd[q + 5] << 8);
+                const int32_t q23 = dd[q + 6] + (int32_t)((uint32_t)dd[q + 7] << 8);
+                const int64_t one = (int32_t)gl::opaque(1u), m16 = (int32_t)gl::opaque(65536u);
+                uint64_t L = gl::vopaque64((uint64_t)((int64_t)p01 * one + (int64_t)k[2 * e]));
+                uint64_t H = gl::vopaque64((uint64_t)((int64_t)q01 * one + (int64_t)k[2 * e + 1]));
+                if (e == 0) {
+                    L = gl::vopaque64((uint64_t)dl * gl::opaque(8u) + L);
+                    H = gl::vopaque64((uint64_t)dh * gl::opaque(8u) + H);
+                }
+                L = gl::vopaque64((uint64_t)((int64_t)p23 * m16 + (int64_t)L));
+                H = gl::vopaque64((uint64_t)((int64_t)q23 * m16 + (int64_t)H));
+                s[t][e] = limb_combine(L, H);
             }
-            L = gl::vopaque64((uint64_t)((int64_t)p23 * m16 + (int64_t)L));
-            H = gl::vopaque64((uint64_t)((int64_t)q23 * m16 + (int64_t)H));
-            s[t][e] = limb_combine(L, H);
         }
     }
 }

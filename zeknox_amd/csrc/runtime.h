@@ -103,12 +103,17 @@ struct Device {
     // caller or found by the autotuner).  Guarded by tune_mu.
     std::mutex tune_mu;
     Tuning tune_default;
+    bool tune_default_set = false;  // the caller replaced the built-in default
     std::map<TuneKey, Tuning> tuned;
     bool autotune_new_shapes = false;  // tune unseen shapes in zkx_commit_batch
     // Batch engines replay launch lists they have issued before as HIP
     // graphs (zkx_set_commit_graphs); read at each commit.
     std::atomic<bool> commit_graphs{true};
     Tuning tuning_default();
+    // The schedule of the C-ABI Merkle entry points for one tree: the
+    // default, with lone_tree_tile_log()'s work-groups unless the caller set
+    // a default.
+    Tuning cabi_merkle_tuning(uint64_t num_leaves, uint64_t hash_type);
     // The shape's entry, or the default (found = false).
     Tuning tuning_for(const TuneKey &k, bool *found = nullptr);
 };

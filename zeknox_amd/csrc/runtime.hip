@@ -107,6 +107,14 @@ Tuning Device::tuning_default()
     return tune_default;
 }
 
+Tuning Device::cabi_merkle_tuning(uint64_t num_leaves, uint64_t hash_type)
+{
+    std::lock_guard<std::mutex> g(tune_mu);
+    Tuning t = tune_default;
+    if (!tune_default_set) t.merkle_tile_log = lone_tree_tile_log(num_leaves, hash_type);
+    return t;
+}
+
 Tuning Device::tuning_for(const TuneKey &k, bool *found)
 {
     std::lock_guard<std::mutex> g(tune_mu);

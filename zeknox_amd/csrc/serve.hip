@@ -248,7 +248,7 @@ bool Server::launch(size_t e, std::vector<ServeRequest *> &batch, std::string &e
         {
             std::lock_guard<std::mutex> g(eng.mu);
             try {
-                eng.commit(jobs.data(), jobs.size(), nullptr, nullptr, lde.data());
+                eng.commit(jobs.data(), jobs.size(), nullptr, nullptr, lde.data(), slot.large ? kLargeMerkleChunk : 0);
             } catch (...) {
                 eng.mark_failed();
                 throw;

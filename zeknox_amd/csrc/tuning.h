@@ -23,6 +23,31 @@ struct Tuning {
     uint32_t merkle_valu = 0;
 };
 
+// Work-group size (log2 lanes) of the C-ABI Merkle entry points for one tree
+// of num_leaves leaves while the caller has not set a default schedule.  A
+// tree alone has no other tree's leaves to hide the upper levels, which run
+// at low occupancy at the end of the launch, so the largest tile that still
+// gives every CU a work-group (2^lg / 2^tile >= 256) wins up to 2^18 leaves;
+// above, leaves take several rounds of work-groups and 2^9 lanes win for
+// Poseidon and Poseidon2, 2^8 for Keccak-256 and Monolith, 2^10 for Poseidon
+// over BN254 (as measured; the cause is not analysed).  Measured on one
+// MI355X, leaf launch of 135-element leaves, cap 4, tiles 2^8 / 2^9 / 2^10:
+//   Poseidon  2^16 1.07 / 1.31 / 2.09 ms, 2^17 1.51 / 1.38 / 2.12,
+//             2^18 2.65 / 2.52 / 2.41, 2^20 9.32 / 9.08 / 9.61;
+//   Keccak    2^17 0.39 / 0.33 / 0.45, 2^18 0.57 / 0.58 / 0.50,
+//             2^20 1.82 / 1.97 / 1.98 (Poseidon2, Monolith alike);
+//   BN254     2^19 178 / 181 / 145, 2^20 324 / 359 / 281.
+// In a batch other trees' leaves hide that tail and the batch default or
+// the autotuned schedule applies.  hash_type: merkle/merkle.h numbering.
+inline uint32_t lone_tree_tile_log(uint64_t num_leaves, uint64_t hash_type)
+{
+    const uint32_t lg = num_leaves ? 63u - (uint32_t)__builtin_clzll(num_leaves) : 0u;
+    if (lg <= 16) return 8;
+    if (lg <= 18) return lg - 8;
+    if (hash_type == 2) return 10;                   // Poseidon over BN254
+    return hash_type == 0 || hash_type == 3 ? 9 : 8;  // Poseidon, Poseidon2
+}
+
 // A commit shape: (columns, log2 degree, rate bits, hash type, leaf rows kept).
 using TuneKey = std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, bool>;
 
