@@ -69,7 +69,7 @@ NTT_TRAFFIC_BYTES_PER_COMMIT = 8.87e8
 #   SQ_INSTS_VALU), keyed by the LDE's pass split (stages per pass), measured
 #   on the sources NTT_COUNTERS_SOURCES
 NTT_VALU_WAVE_INSTS_PER_COMMIT = {(3, 12): 1.4536e8, (4, 11): 1.4735e8}
-NTT_COUNTERS_SOURCES = "a90afaf1241ae432"
+NTT_COUNTERS_SOURCES = "693c53246da7f518"
 # 32-bit VALU issue peak: 256 CUs x 4 SIMDs x one wave64 instruction per 4
 # cycles x 2.4 GHz = 39.3 T lane-ops/s (MI355X's published 78.6 TFLOP/s
 # vector FP64 counts an FMA as 2 flops on the same issue rate)

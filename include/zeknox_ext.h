@@ -113,6 +113,16 @@ typedef struct {
 /* All jobs of one call run together (one launch per pass over all jobs);
  * returns when they are complete. */
 RustError zkx_commit_batch(size_t device_id, const ZkxCommitJob *jobs, size_t njobs);
+/* zkx_commit_batch with its long launches split: every NTT pass launch in
+ * ranges of at most `ntt_tiles` 4096-element tiles and every Merkle launch
+ * in ranges of at most `merkle_groups` work-groups (0: one launch), in
+ * order on the same stream.  For callers that share the GPU with
+ * latency-sensitive work on other streams: the device dispatches another
+ * stream's kernels between two launches, not while a launch still has
+ * work-groups to dispatch or its persistent work-groups hold every CU (the
+ * commit server runs its large requests this way).  Same results. */
+RustError zkx_commit_batch_pieces(size_t device_id, const ZkxCommitJob *jobs, size_t njobs, uint64_t ntt_tiles,
+                                  uint64_t merkle_groups);
 /* Device working memory zkx_commit_batch needs for these jobs, in bytes.
  * The library keeps it between calls (one buffer per device, grown as
  * needed); zkx_commit_trim frees it. */

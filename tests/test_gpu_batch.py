@@ -45,6 +45,31 @@ MIX = [(2, 10, 1, 4, 0), (20, 11, 2, 4, 3), (80, 10, 3, 4, 0), (135, 9, 1, 4, 3)
        (17, 9, 2, 4, 1), (34, 8, 1, 3, 4), (3, 7, 1, 2, 1), (12, 6, 3, 4, 4), (9, 6, 1, 3, 2), (5, 4, 2, 1, 2)]
 
 
+# the MIX plus strided passes of 5..7 stages (one tile per work-group) and
+# a 2^19-leaf tree
+PIECES_MIX = MIX + [(8, 15, 2, 4, 0), (6, 15, 3, 4, 3), (4, 16, 3, 4, 0)]
+
+
+@pytest.mark.parametrize("ntt_tiles,merkle_groups", [(1, 1), (5, 3), (64, 7), (0, 2), (3, 0)])
+def test_batch_in_pieces_vs_oracle(zk, ntt_tiles, merkle_groups):
+    """zkx_commit_batch_pieces: every NTT pass launch split into ranges of
+    ntt_tiles tiles and every Merkle launch into ranges of merkle_groups
+    work-groups gives the oracle's results (and so the unsplit batch's)."""
+    commits, inputs = [], []
+    for i, (cols, lg_n, rate, h, ht) in enumerate(PIECES_MIX):
+        x = ob.canonical(cols << lg_n, 900 + i)
+        inputs.append(x)
+        commits.append(zk.Commit(x, cols, lg_n, rate, h, ht))
+    print(f"pieces ntt_tiles={ntt_tiles} merkle_groups={merkle_groups}", flush=True)
+    zk.commit_batch(commits, ntt_tiles=ntt_tiles, merkle_groups=merkle_groups)
+    for (cols, lg_n, rate, h, ht), x, c in zip(PIECES_MIX, inputs, commits):
+        leaves, dig, cap = c.results()
+        el, ed, ec = oracle_commit(x, cols, lg_n, rate, h, ht)
+        assert np.array_equal(leaves, el), f"leaves cols={cols} lg_n={lg_n} rate={rate}"
+        assert np.array_equal(dig, ed), f"digests cols={cols} lg_n={lg_n} rate={rate} ht={ht}"
+        assert np.array_equal(cap, ec)
+
+
 def test_batch_mixed_vs_oracle(zk):
     commits, inputs = [], []
     for i, (cols, lg_n, rate, h, ht) in enumerate(MIX):

@@ -173,6 +173,7 @@ def _load():
         "zkx_kernel_stats": (ctypes.c_int, [vp, vp, vp, ctypes.c_int]),
         "zkx_kernel_stats_reset": (None, []),
         "zkx_commit_batch": (RustError, [sz, ctypes.POINTER(CommitJob), sz]),
+        "zkx_commit_batch_pieces": (RustError, [sz, ctypes.POINTER(CommitJob), sz, ctypes.c_uint64, ctypes.c_uint64]),
         "zkx_commit_scratch_bytes": (sz, [ctypes.POINTER(CommitJob), sz]),
         "zkx_server_create": (RustError, [sz, sz, ctypes.c_int, sz, ctypes.POINTER(vp)]),
         "zkx_server_destroy": (RustError, [vp]),
@@ -418,10 +419,15 @@ def gl_ops(op, a, b, device_id=0):
     return out.to_host()
 
 
-def commit_batch(commits, device_id=0):
-    """Run a list of Commit objects as one batch (synchronous)."""
+def commit_batch(commits, device_id=0, ntt_tiles=0, merkle_groups=0):
+    """Run a list of Commit objects as one batch (synchronous); ntt_tiles /
+    merkle_groups > 0 split the long launches into ranges of that size
+    (zkx_commit_batch_pieces)."""
     arr = (CommitJob * len(commits))(*[c.job() for c in commits])
-    _check(lib.zkx_commit_batch(device_id, arr, len(commits)))
+    if ntt_tiles or merkle_groups:
+        _check(lib.zkx_commit_batch_pieces(device_id, arr, len(commits), ntt_tiles, merkle_groups))
+    else:
+        _check(lib.zkx_commit_batch(device_id, arr, len(commits)))
 
 
 def commit_scratch_bytes(commits):

@@ -35,6 +35,13 @@ struct CommitJob {
     uint32_t cols = 0, lg_n = 0, rate = 0, cap_height = 0, hash_type = 0;
 };
 
+// Long launches of a commit issued as several launches over ranges of at
+// most this many NTT tiles / Merkle work-groups (0: one launch each).
+struct LaunchPieces {
+    uint64_t ntt_tiles = 0;
+    uint64_t merkle_groups = 0;
+};
+
 void check_commit_job(const CommitJob &j);
 // Device bytes of working memory an Engine needs for these jobs.
 size_t commit_scratch_bytes(const CommitJob *jobs, size_t n);
@@ -55,11 +62,11 @@ public:
     // `forced`, when given, replaces every job's schedule parameters.
     // `lde_bufs`, when given, holds one working buffer per job instead
     // (job i: cols << (lg_n + rate) elements, 256-byte aligned).
-    // `merkle_chunk` > 0: every Merkle launch is issued in pieces of at most
-    // that many work-groups (a large served request, so that other streams'
-    // kernels are dispatched between the pieces).
+    // `pieces`: long launches issued as several launches over ranges (a
+    // large served request, so that other streams' kernels are dispatched
+    // between them).
     void commit(const CommitJob *jobs, size_t n, void *scratch = nullptr, const Tuning *forced = nullptr,
-                uint64_t *const *lde_bufs = nullptr, uint64_t merkle_chunk = 0);
+                uint64_t *const *lde_bufs = nullptr, const LaunchPieces &pieces = LaunchPieces{});
     std::mutex mu;
     void sync();
     // After a failed commit or wait: the fused Merkle launches' arrival
@@ -89,7 +96,7 @@ private:
         int pipe;           // NTT: ntt_pipe_kind of the list
         uint32_t tile_log;  // Merkle work-group size (log2 lanes)
     };
-    void issue(hipStream_t s, const std::vector<Launch> &launches, uint64_t merkle_chunk);
+    void issue(hipStream_t s, const std::vector<Launch> &launches, const LaunchPieces &pieces);
     void drop_graphs();
 
     Device &dev_;

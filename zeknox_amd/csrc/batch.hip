@@ -74,7 +74,7 @@ void Engine::trim()
 }
 
 void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning *forced, uint64_t *const *lde_bufs,
-                    uint64_t merkle_chunk)
+                    const LaunchPieces &pieces)
 {
     if (n == 0) return;
     for (size_t i = 0; i < n; i++) check_commit_job(jobs[i]);
@@ -250,9 +250,10 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
     try {
         if (use_graphs_ && dev_.commit_graphs && !profiling() && !forced) {  // (the autotuner times direct launches)
             std::vector<uint64_t> key;
-            key.reserve(2 + 7 * launches.size());
+            key.reserve(3 + 7 * launches.size());
             key.push_back((uint64_t)(uintptr_t)desc_.p);
-            key.push_back(merkle_chunk);
+            key.push_back(pieces.ntt_tiles);
+            key.push_back(pieces.merkle_groups);
             for (const Launch &l : launches)
                 key.insert(key.end(), {(uint64_t)l.offset, l.count, l.blocks, (uint64_t)l.kind, (uint64_t)l.hash,
                                        (uint64_t)l.pipe, l.tile_log});
@@ -268,7 +269,7 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
                 hipError_t ce = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
                 if (ce == hipSuccess) {
                     try {
-                        issue(cap, launches, merkle_chunk);
+                        issue(cap, launches, pieces);
                     } catch (...) {
                         hipGraph_t g = nullptr;
                         (void)hipStreamEndCapture(cap, &g);
@@ -292,23 +293,23 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
                 return;
             }
         }
-        issue(stream_, launches, merkle_chunk);
+        issue(stream_, launches, pieces);
     } catch (...) {
         ctr_dirty_ = true;
         throw;
     }
 }
 
-void Engine::issue(hipStream_t s, const std::vector<Launch> &launches, uint64_t merkle_chunk)
+void Engine::issue(hipStream_t s, const std::vector<Launch> &launches, const LaunchPieces &pieces)
 {
     for (const Launch &l : launches) {
         const char *base = static_cast<const char *>(desc_.p) + l.offset;
         if (l.kind == 0) {
             launch_ntt_list(s, reinterpret_cast<const NttPass *>(base), l.count, l.blocks, l.hash == 1, l.pipe,
-                            "batch_ntt");
+                            "batch_ntt", pieces.ntt_tiles);
         } else {
             launch_merkle_pass(s, l.hash, reinterpret_cast<const MerkleArgs *>(base), l.count, MerkleArgs{},
-                               l.blocks, l.tile_log, merkle_chunk);
+                               l.blocks, l.tile_log, pieces.merkle_groups);
         }
     }
 }

@@ -336,13 +336,19 @@ extern "C" {
 
 RustError zkx_commit_batch(size_t device_id, const ZkxCommitJob *jobs, size_t njobs)
 {
+    return zkx_commit_batch_pieces(device_id, jobs, njobs, 0, 0);
+}
+
+RustError zkx_commit_batch_pieces(size_t device_id, const ZkxCommitJob *jobs, size_t njobs, uint64_t ntt_tiles,
+                                  uint64_t merkle_groups)
+{
     return guarded([&] {
         std::vector<CommitJob> v = to_jobs(jobs, njobs);
         Device &d = device(device_id);
         Engine &e = default_engine(d);
         std::lock_guard<std::mutex> g(e.mu);
         try {
-            e.commit(v.data(), v.size());
+            e.commit(v.data(), v.size(), nullptr, nullptr, nullptr, LaunchPieces{ntt_tiles, merkle_groups});
             e.sync();
         } catch (...) {
             e.mark_failed();

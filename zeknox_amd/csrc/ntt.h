@@ -49,9 +49,11 @@ int ntt_pipe_kind(const NttPass *host_list, size_t n);
 
 // One launch over n pass descriptors in device memory (sorted by
 // tile_begin), all contiguous (s_lo == 0) or all strided; `pipe_kind` as
-// ntt_pipe_kind of the same list.
+// ntt_pipe_kind of the same list.  chunk > 0: launches over ranges of at
+// most `chunk` tiles instead, in order (other streams' kernels can be
+// dispatched between them).
 void launch_ntt_list(hipStream_t stream, const NttPass *dev_list, uint32_t n, uint64_t blocks, bool contiguous,
-                     int pipe_kind, const char *name);
+                     int pipe_kind, const char *name, uint64_t chunk = 0);
 
 // table[i] = base^i, i < n.
 void fill_powers(uint64_t *dst, uint64_t n, uint64_t base, hipStream_t s);

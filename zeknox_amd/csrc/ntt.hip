@@ -36,39 +36,40 @@ uint64_t resident_groups(Kernel kernel)
     return n;
 }
 
+// A persistent launch over the tiles [first, end) of the list.
 template <class Kernel>
 void launch_persistent(Kernel kernel, hipStream_t stream, const NttPass *dev_list, uint32_t n, const NttPass &one,
-                       uint64_t tiles)
+                       uint64_t first, uint64_t end)
 {
-    const uint64_t groups = std::min(tiles, resident_groups(kernel));
-    hipLaunchKernelGGL(kernel, dim3((uint32_t)groups), dim3(kNttThreads), 0, stream, dev_list, n, one, tiles);
+    const uint64_t groups = std::min(end - first, resident_groups(kernel));
+    hipLaunchKernelGGL(kernel, dim3((uint32_t)groups), dim3(kNttThreads), 0, stream, dev_list, n, one, end, first);
 }
 
 template <int MODE>
 void launch_strided_pipe(int m, hipStream_t stream, const NttPass *dev_list, uint32_t n, const NttPass &one,
-                         uint64_t tiles)
+                         uint64_t first, uint64_t end)
 {
     switch (m) {
-    case 1: launch_persistent(k_ntt_strided_pipe<MODE, 1>, stream, dev_list, n, one, tiles); break;
-    case 2: launch_persistent(k_ntt_strided_pipe<MODE, 2>, stream, dev_list, n, one, tiles); break;
-    case 3: launch_persistent(k_ntt_strided_pipe<MODE, 3>, stream, dev_list, n, one, tiles); break;
-    default: launch_persistent(k_ntt_strided_pipe<MODE, 4>, stream, dev_list, n, one, tiles); break;
+    case 1: launch_persistent(k_ntt_strided_pipe<MODE, 1>, stream, dev_list, n, one, first, end); break;
+    case 2: launch_persistent(k_ntt_strided_pipe<MODE, 2>, stream, dev_list, n, one, first, end); break;
+    case 3: launch_persistent(k_ntt_strided_pipe<MODE, 3>, stream, dev_list, n, one, first, end); break;
+    default: launch_persistent(k_ntt_strided_pipe<MODE, 4>, stream, dev_list, n, one, first, end); break;
     }
 }
 
-// A launch over the passes (device list, or `one` when n == 0) with the
-// kernel ntt_pipe_kind chose; false for kind 0.
+// A launch over the tiles [first, end) of the passes (device list, or `one`
+// when n == 0) with the kernel ntt_pipe_kind chose; false for kind 0.
 bool launch_pipe(int kind, hipStream_t stream, const NttPass *dev_list, uint32_t n, const NttPass &one,
-                 uint64_t tiles)
+                 uint64_t first, uint64_t end)
 {
     if (kind == 1) {
-        launch_persistent(k_ntt_contig_pipe, stream, dev_list, n, one, tiles);
+        launch_persistent(k_ntt_contig_pipe, stream, dev_list, n, one, first, end);
         return true;
     }
     if (kind >= 2 && kind < 10) {
         const int m = (kind - 2) % 4 + 1;
-        if (kind >= 6) launch_strided_pipe<kSrcLde>(m, stream, dev_list, n, one, tiles);
-        else launch_strided_pipe<kSrcInPlace>(m, stream, dev_list, n, one, tiles);
+        if (kind >= 6) launch_strided_pipe<kSrcLde>(m, stream, dev_list, n, one, first, end);
+        else launch_strided_pipe<kSrcInPlace>(m, stream, dev_list, n, one, first, end);
         return true;
     }
     return false;
@@ -79,13 +80,13 @@ void launch_pass(Device &d, const NttPass &p, uint64_t tiles, const char *name)
 {
     if (tiles > 0x7FFFFFFFull) invalid("transform too large for one launch");
     ProfScope ps(name, d.stream);
-    if (launch_pipe(ntt_pipe_kind(&p, 1), d.stream, nullptr, 0u, p, tiles)) {
+    if (launch_pipe(ntt_pipe_kind(&p, 1), d.stream, nullptr, 0u, p, 0, tiles)) {
     } else if (p.s_lo == 0)
         hipLaunchKernelGGL(k_ntt_tiles<true>, dim3((uint32_t)tiles), dim3(kNttThreads), 0, d.stream,
-                           (const NttPass *)nullptr, 0u, p);
+                           (const NttPass *)nullptr, 0u, p, (uint64_t)0);
     else
         hipLaunchKernelGGL(k_ntt_tiles<false>, dim3((uint32_t)tiles), dim3(kNttThreads), 0, d.stream,
-                           (const NttPass *)nullptr, 0u, p);
+                           (const NttPass *)nullptr, 0u, p, (uint64_t)0);
     ZKX_HIP(hipGetLastError());
 }
 
@@ -124,20 +125,23 @@ int ntt_pipe_kind(const NttPass *host_list, size_t n)
 }
 
 void launch_ntt_list(hipStream_t stream, const NttPass *dev_list, uint32_t n, uint64_t blocks, bool contiguous,
-                     int pipe_kind, const char *name)
+                     int pipe_kind, const char *name, uint64_t chunk)
 {
     if (blocks == 0) return;
     if (blocks > 0x7FFFFFFFull) invalid("transform batch too large for one launch");
     ProfScope ps(name, stream);
-    if (launch_pipe(pipe_kind, stream, dev_list, n, NttPass{}, blocks)) {
+    const uint64_t step = chunk ? chunk : blocks;
+    for (uint64_t first = 0; first < blocks; first += step) {
+        const uint64_t end = std::min(blocks, first + step);
+        if (launch_pipe(pipe_kind, stream, dev_list, n, NttPass{}, first, end)) {
+        } else if (contiguous)
+            hipLaunchKernelGGL(k_ntt_tiles<true>, dim3((uint32_t)(end - first)), dim3(kNttThreads), 0, stream,
+                               dev_list, n, NttPass{}, first);
+        else
+            hipLaunchKernelGGL(k_ntt_tiles<false>, dim3((uint32_t)(end - first)), dim3(kNttThreads), 0, stream,
+                               dev_list, n, NttPass{}, first);
+        ZKX_HIP(hipGetLastError());
     }
-    else if (contiguous)
-        hipLaunchKernelGGL(k_ntt_tiles<true>, dim3((uint32_t)blocks), dim3(kNttThreads), 0, stream, dev_list, n,
-                           NttPass{});
-    else
-        hipLaunchKernelGGL(k_ntt_tiles<false>, dim3((uint32_t)blocks), dim3(kNttThreads), 0, stream, dev_list, n,
-                           NttPass{});
-    ZKX_HIP(hipGetLastError());
 }
 
 // Stage counts of the passes, top stages first: every pass but the last is

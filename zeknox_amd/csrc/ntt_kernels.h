@@ -263,22 +263,24 @@ __device__ __forceinline__ void ntt_round_last(uint64_t *lds, uint32_t t, bool i
 // `passes` (npasses > 0, device memory) lists independent passes sorted by
 // tile_begin; with npasses == 0 the launch runs the single pass `one`.
 // CONTIG: every pass of the launch is contiguous (s_lo == 0), else strided.
+// Work-group k runs tile first + k of the list (a launch may cover a range).
 // Four work-groups per CU: 33 KB of LDS and at most 128 VGPRs each.
 template <bool CONTIG>
 __global__ void __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_eu(4)))
-k_ntt_tiles(const NttPass *__restrict__ passes, uint32_t npasses, NttPass one)
+k_ntt_tiles(const NttPass *__restrict__ passes, uint32_t npasses, NttPass one, uint64_t first)
 {
     __shared__ uint64_t lds[(1 << kNttTileLog) + (1 << kNttTileLog) / 32];
 
     // locate this work-group's pass (uniform)
+    const uint64_t bid = first + blockIdx.x;
     uint32_t lo = 0, hi = npasses;
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (passes[mid].tile_begin <= blockIdx.x) lo = mid;
+        if (passes[mid].tile_begin <= bid) lo = mid;
         else hi = mid;
     }
     const NttPass a = npasses ? passes[lo] : one;
-    const uint64_t tile = blockIdx.x - a.tile_begin;
+    const uint64_t tile = bid - a.tile_begin;
     const uint32_t t = threadIdx.x;
 
     // ---- load, coalesced: lane t takes positions t + 256 i, all 16 in flight,
@@ -369,7 +371,8 @@ k_ntt_tiles(const NttPass *__restrict__ passes, uint32_t npasses, NttPass one)
 //
 // A contiguous pass is load -> up to 12 stages in LDS -> store per tile; a
 // work-group that runs one tile and exits leaves HBM idle while it computes.
-// Here each work-group walks tiles blockIdx.x, +gridDim.x, ... and issues the
+// Here each work-group walks tiles first + blockIdx.x, +gridDim.x, ... (up to
+// ntiles: a launch covers the tiles [first, ntiles) of its list) and issues the
 // global loads of its next tile before the rounds of the current one, so the
 // loads (16 per lane, in registers) are in flight while it computes.  The
 // stage twiddles come from an LDS copy of the w_4096 table, loaded once per
@@ -443,15 +446,15 @@ __device__ __forceinline__ void contig_rounds(uint64_t *lds, const NttPass &a, u
     if (rem) __syncthreads();
 }
 
-// Launch with at most `ntiles` work-groups over passes that satisfy
+// Launch with at most ntiles - first work-groups over passes that satisfy
 // ntt_pipe_ok.
 __global__ void __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_eu(kNttPipeWaves)))
-k_ntt_contig_pipe(const NttPass *__restrict__ passes, uint32_t npasses, NttPass one, uint64_t ntiles)
+k_ntt_contig_pipe(const NttPass *__restrict__ passes, uint32_t npasses, NttPass one, uint64_t ntiles, uint64_t first)
 {
     __shared__ uint64_t lds[(1 << kNttTileLog) + (1 << kNttTileLog) / 32];
     __shared__ uint64_t twl[kTw12Half];
     const uint32_t t = threadIdx.x;
-    const uint64_t b = blockIdx.x;
+    const uint64_t b = first + blockIdx.x;
     if (b >= ntiles) return;
     NttPass a = pass_at(passes, npasses, one, b);
     uint64_t tile = b - a.tile_begin;
@@ -636,11 +639,12 @@ __device__ __forceinline__ void strided_fetch(const NttPass &a, uint64_t tile, u
 
 template <int MODE, int K>
 __global__ void __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_eu(kNttPipeWaves)))
-k_ntt_strided_pipe(const NttPass *__restrict__ passes, uint32_t npasses, NttPass one, uint64_t ntiles)
+k_ntt_strided_pipe(const NttPass *__restrict__ passes, uint32_t npasses, NttPass one, uint64_t ntiles,
+                   uint64_t first)
 {
     __shared__ uint64_t lds[(1 << kNttTileLog) + (1 << kNttTileLog) / 32];
     const uint32_t t = threadIdx.x;
-    const uint64_t b = blockIdx.x;
+    const uint64_t b = first + blockIdx.x;
     if (b >= ntiles) return;
     NttPass a = pass_at(passes, npasses, one, b);
     uint64_t tile = b - a.tile_begin;

@@ -248,7 +248,9 @@ bool Server::launch(size_t e, std::vector<ServeRequest *> &batch, std::string &e
         {
             std::lock_guard<std::mutex> g(eng.mu);
             try {
-                eng.commit(jobs.data(), jobs.size(), nullptr, nullptr, lde.data(), slot.large ? kLargeMerkleChunk : 0);
+                LaunchPieces pieces;
+                if (slot.large) pieces = LaunchPieces{kLargeNttChunk, kLargeMerkleChunk};
+                eng.commit(jobs.data(), jobs.size(), nullptr, nullptr, lde.data(), pieces);
             } catch (...) {
                 eng.mark_failed();
                 throw;
