@@ -61,6 +61,31 @@ def test_ntt_vs_oracle(zk, lg, B, inverse, coset):
     assert np.array_equal(buf, ob.ntt(x, lg, B, inverse, coset))
 
 
+# First passes of 5..8 stages (the two-round strided kernel) over more tiles
+# than the device holds work-groups, so every work-group walks several tiles
+# (prefetch and twiddle reloads); in place (NTT, with the coset it reads
+# through the one-tile kernel) and from coefficients (LDE).
+@pytest.mark.parametrize("lg,B", [(17, 40), (18, 16), (19, 8), (20, 8)])
+@pytest.mark.parametrize("inverse", [False, True])
+@pytest.mark.parametrize("coset", [False, True])
+def test_ntt_two_round_first_pass_vs_oracle(zk, lg, B, inverse, coset):
+    print(f"ntt lg={lg} B={B} inverse={inverse} coset={coset}", flush=True)
+    x = ob.canonical(B << lg, 11 * lg + B + 2 * inverse + coset)
+    buf = x.copy()
+    (zk.intt_batch if inverse else zk.ntt_batch)(0, buf, lg, zk.NTTConfig(batches=B, with_coset=coset))
+    assert np.array_equal(buf, ob.ntt(x, lg, B, inverse, coset))
+
+
+@pytest.mark.parametrize("lg_n,rate,B", [(17, 1, 8), (18, 2, 4), (19, 1, 4), (20, 1, 4)])
+@pytest.mark.parametrize("coset", [True, False])
+def test_lde_two_round_first_pass_vs_oracle(zk, lg_n, rate, B, coset):
+    print(f"lde lg_n={lg_n} rate={rate} B={B} coset={coset}", flush=True)
+    x = ob.canonical(B << lg_n, 13 * lg_n + rate + B)
+    out = np.zeros(B << (lg_n + rate), np.uint64)
+    zk.lde_batch(0, out, x, lg_n, zk.NTTConfig(batches=B, extension_rate_bits=rate, with_coset=coset))
+    assert np.array_equal(out, ob.lde(x, lg_n, rate, B, coset=coset))
+
+
 def test_ntt_roundtrip_on_device(zk):
     lg, B = 18, 2
     x = ob.canonical(B << lg, 42)

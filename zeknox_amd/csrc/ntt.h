@@ -44,7 +44,9 @@ std::vector<uint32_t> ntt_pass_sizes(uint32_t L, const Tuning &tun);
 // Which persistent, pipelined kernel a launch over these passes can use:
 // 0 none; 1 contiguous (every pass in place and a whole number of tiles);
 // 2 + 4 * lde + (m - 1) strided single-round (every pass strided with the
-// same m <= 4 stages and the same source, in place or LDE).
+// same m <= 4 stages and the same source, in place or LDE, persistent and
+// pipelined); 10 + 4 * lde + (m - 5) strided two-round (the same with
+// 5 <= m <= 8, one tile per work-group, first round in registers).
 int ntt_pipe_kind(const NttPass *host_list, size_t n);
 
 // One launch over n pass descriptors in device memory (sorted by

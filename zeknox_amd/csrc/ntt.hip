@@ -45,15 +45,31 @@ void launch_persistent(Kernel kernel, hipStream_t stream, const NttPass *dev_lis
     hipLaunchKernelGGL(kernel, dim3((uint32_t)groups), dim3(kNttThreads), 0, stream, dev_list, n, one, end, first);
 }
 
+// One work-group per tile of [first, end).
 template <int MODE>
-void launch_strided_pipe(int m, hipStream_t stream, const NttPass *dev_list, uint32_t n, const NttPass &one,
-                         uint64_t first, uint64_t end)
+void launch_strided1(int m, hipStream_t stream, const NttPass *dev_list, uint32_t n, const NttPass &one,
+                     uint64_t first, uint64_t end)
 {
+    const dim3 grid((uint32_t)(end - first)), block(kNttThreads);
     switch (m) {
-    case 1: launch_persistent(k_ntt_strided_pipe<MODE, 1>, stream, dev_list, n, one, first, end); break;
-    case 2: launch_persistent(k_ntt_strided_pipe<MODE, 2>, stream, dev_list, n, one, first, end); break;
-    case 3: launch_persistent(k_ntt_strided_pipe<MODE, 3>, stream, dev_list, n, one, first, end); break;
-    default: launch_persistent(k_ntt_strided_pipe<MODE, 4>, stream, dev_list, n, one, first, end); break;
+    case 1: hipLaunchKernelGGL((k_ntt_strided1_tiles<MODE, 1>), grid, block, 0, stream, dev_list, n, one, first); break;
+    case 2: hipLaunchKernelGGL((k_ntt_strided1_tiles<MODE, 2>), grid, block, 0, stream, dev_list, n, one, first); break;
+    case 3: hipLaunchKernelGGL((k_ntt_strided1_tiles<MODE, 3>), grid, block, 0, stream, dev_list, n, one, first); break;
+    default: hipLaunchKernelGGL((k_ntt_strided1_tiles<MODE, 4>), grid, block, 0, stream, dev_list, n, one, first); break;
+    }
+}
+
+// One work-group per tile of [first, end).
+template <int MODE>
+void launch_strided2(int m, hipStream_t stream, const NttPass *dev_list, uint32_t n, const NttPass &one,
+                     uint64_t first, uint64_t end)
+{
+    const dim3 grid((uint32_t)(end - first)), block(kNttThreads);
+    switch (m) {
+    case 5: hipLaunchKernelGGL((k_ntt_strided2_tiles<MODE, 5>), grid, block, 0, stream, dev_list, n, one, first); break;
+    case 6: hipLaunchKernelGGL((k_ntt_strided2_tiles<MODE, 6>), grid, block, 0, stream, dev_list, n, one, first); break;
+    case 7: hipLaunchKernelGGL((k_ntt_strided2_tiles<MODE, 7>), grid, block, 0, stream, dev_list, n, one, first); break;
+    default: hipLaunchKernelGGL((k_ntt_strided2_tiles<MODE, 8>), grid, block, 0, stream, dev_list, n, one, first); break;
     }
 }
 
@@ -68,8 +84,14 @@ bool launch_pipe(int kind, hipStream_t stream, const NttPass *dev_list, uint32_t
     }
     if (kind >= 2 && kind < 10) {
         const int m = (kind - 2) % 4 + 1;
-        if (kind >= 6) launch_strided_pipe<kSrcLde>(m, stream, dev_list, n, one, first, end);
-        else launch_strided_pipe<kSrcInPlace>(m, stream, dev_list, n, one, first, end);
+        if (kind >= 6) launch_strided1<kSrcLde>(m, stream, dev_list, n, one, first, end);
+        else launch_strided1<kSrcInPlace>(m, stream, dev_list, n, one, first, end);
+        return true;
+    }
+    if (kind >= 10 && kind < 18) {
+        const int m = (kind - 10) % 4 + 5;
+        if (kind >= 14) launch_strided2<kSrcLde>(m, stream, dev_list, n, one, first, end);
+        else launch_strided2<kSrcInPlace>(m, stream, dev_list, n, one, first, end);
         return true;
     }
     return false;
@@ -116,11 +138,12 @@ int ntt_pipe_kind(const NttPass *host_list, size_t n)
         }
         return 1;
     }
-    if (p0.m < 1 || p0.m > 4 || (p0.mode != kSrcInPlace && p0.mode != kSrcLde)) return 0;
+    if (p0.m < 1 || p0.m > 8 || (p0.mode != kSrcInPlace && p0.mode != kSrcLde)) return 0;
     for (size_t i = 0; i < n; i++) {
         const NttPass &p = host_list[i];
         if (p.s_lo == 0 || p.m != p0.m || p.mode != p0.mode) return 0;
     }
+    if (p0.m >= 5) return 10 + (p0.mode == kSrcLde ? 4 : 0) + (int)(p0.m - 5);
     return 2 + (p0.mode == kSrcLde ? 4 : 0) + (int)(p0.m - 1);
 }
 

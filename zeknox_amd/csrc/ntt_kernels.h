@@ -511,21 +511,20 @@ k_ntt_contig_pipe(const NttPass *__restrict__ passes, uint32_t npasses, NttPass 
     }
 }
 
-// ---- strided single-round passes, persistent and pipelined
+// ---- strided single-round passes, one tile per work-group, registers only
 //
-// A strided pass of m <= 4 stages is one register round (b0 = 0, RB = c =
-// 12 - m >= 8): lane t holds u = (rb << c) | o, o = t | g << 8 (rb < 2^m,
-// g < 2^(4-m)), so the twiddle of stage dd for rb is
+// A strided pass of m = K <= 4 stages is one register round (b0 = 0, RB = c =
+// 12 - K >= 8): lane t's loaded positions t + 256 i are u = (rb << c) | o,
+// o = t | g << 8, i = rb << (4 - K) | g (rb < 2^K, g < 2^(4-K)), so every
+// butterfly group of the round lies in one lane's registers: no LDS and no
+// barrier.  The twiddle of stage dd for rb is
 //   W(dd, g, v) = w_{2^(s_lo+dd+1)}^((v << s_lo) | (lo_base + o)),  v = rb mod 2^dd,
-// at most 15 values per lane, all known once the tile is known.  The next
-// tile's twiddles and input factors are loaded before this tile's stores and
-// its input words prefetched after them, so no wait covers a store issued
-// less than a tile ago or the prefetch (vector-memory counters retire in
-// order).  Per tile:
-//   round(k) | twiddles(k+1) | per position: read result(k), write
-//   input(k+1) [x factor], store result(k) | prefetch(k+2) | barrier
-// MODE is the pass's source (kSrcInPlace or kSrcLde), K its stage count; a
-// launch takes passes of one mode and one stage count (ntt_pipe_kind).
+// at most 15 values per lane, loaded with the tile's words.  A persistent
+// form (next tile prefetched, the tile staged in LDS) held three waves per
+// SIMD and ran the 2^18 x 135 LDE pass 10% slower than this one at four or
+// five.  MODE is the pass's source (kSrcInPlace or kSrcLde), K its stage
+// count; a launch takes passes of one mode and one stage count
+// (ntt_pipe_kind).
 
 // tile_step of a strided pass with m = K <= 4 stages (c = 12 - K >= 8): the
 // branches of the general form decided at compile time.
@@ -571,38 +570,6 @@ __device__ __forceinline__ void strided_tw_load(const NttPass &a, uint64_t tile,
     }
 }
 
-// The round itself (ntt_round / ntt_round4 with b0 = 0 on a strided pass),
-// twiddles from registers.
-template <int K>
-__device__ __forceinline__ void strided_round(uint64_t *lds, uint32_t t, uint32_t c,
-                                              const uint64_t (&w)[strided_ntw<K>()])
-{
-    constexpr int E = kNttPerThread;
-    uint64_t x[E];
-    uint32_t sl[E];
-#pragma unroll
-    for (int e = 0; e < E; e++) {
-        const uint32_t rb = (uint32_t)e & ((1u << K) - 1), g = (uint32_t)e >> K;
-        sl[e] = lds_slot((rb << c) | t | (g << 8));
-        x[e] = lds[sl[e]];
-    }
-#pragma unroll
-    for (int dd = K - 1; dd >= 0; dd--) {
-#pragma unroll
-        for (int e = 0; e < E; e++) {
-            if ((e >> dd) & 1) continue;
-            const int f = e | (1 << dd);
-            const int rb = e & ((1 << K) - 1), g = e >> K;
-            const uint64_t wv = w[strided_tw_off<K>(dd) + (g << dd) + (rb & ((1 << dd) - 1))];
-            const uint64_t y0 = x[e], y1 = x[f];
-            x[e] = gl::add(y0, y1);
-            x[f] = gl::mul(gl::sub(y0, y1), wv);
-        }
-    }
-#pragma unroll
-    for (int e = 0; e < E; e++) lds[sl[e]] = x[e];
-}
-
 // Row pointers of a strided tile: the buffer, and for an LDE pass the
 // coefficient row and factor row (the tile lies in one block of 2^L).
 struct StridedRows {
@@ -628,89 +595,221 @@ __device__ __forceinline__ StridedRows strided_rows(const NttPass &a, uint64_t t
     return r;
 }
 
+// Work-group k runs tile first + k of the list.
 template <int MODE, int K>
-__device__ __forceinline__ void strided_fetch(const NttPass &a, uint64_t tile, uint32_t t,
-                                              uint64_t (&raw)[kNttPerThread])
+__global__ void __launch_bounds__(kNttThreads)
+k_ntt_strided1_tiles(const NttPass *__restrict__ passes, uint32_t npasses, NttPass one, uint64_t first)
 {
+    constexpr int CB = 4 - K;
+    const uint32_t t = threadIdx.x;
+    const uint64_t b = first + blockIdx.x;
+    const NttPass a = pass_at(passes, npasses, one, b);
+    const uint64_t tile = b - a.tile_begin;
+    uint64_t w[strided_ntw<K>()];
+    strided_tw_load<K>(a, tile, t, w);
     const StridedRows r = strided_rows<MODE>(a, tile, t);
+    uint64_t x[kNttPerThread];
 #pragma unroll
-    for (int i = 0; i < kNttPerThread; i++) raw[i] = r.src[strided_step<K>(a, i)];
+    for (int i = 0; i < kNttPerThread; i++) x[i] = r.src[strided_step<K>(a, i)];
+    if constexpr (MODE == kSrcLde) {
+#pragma unroll
+        for (int i = 0; i < kNttPerThread; i++) x[i] = gl::mul(x[i], r.fac[strided_step<K>(a, i)]);
+    }
+#pragma unroll
+    for (int dd = K - 1; dd >= 0; dd--) {
+#pragma unroll
+        for (int i = 0; i < kNttPerThread; i++) {
+            const int rb = i >> CB, g = i & ((1 << CB) - 1);
+            if ((rb >> dd) & 1) continue;
+            const int f = i | (1 << (dd + CB));
+            const uint64_t wv = w[strided_tw_off<K>(dd) + (g << dd) + (rb & ((1 << dd) - 1))];
+            const uint64_t y0 = x[i], y1 = x[f];
+            x[i] = gl::add(y0, y1);
+            x[f] = gl::mul(gl::sub(y0, y1), wv);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kNttPerThread; i++) r.buf[strided_step<K>(a, i)] = x[i];
 }
 
-template <int MODE, int K>
-__global__ void __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_eu(kNttPipeWaves)))
-k_ntt_strided_pipe(const NttPass *__restrict__ passes, uint32_t npasses, NttPass one, uint64_t ntiles,
-                   uint64_t first)
+// ---- strided two-round passes, one tile per work-group
+//
+// A strided pass of m = M in 5..8 stages has c = 12 - M < 8 columns per
+// tile: lane t loads positions u = t + 256 i, i.e. column q = t mod 2^c and
+// butterfly index j = (i << (M - 4)) | j0, j0 = t >> c.  Its 16 registers
+// therefore already hold the 16 elements that differ in the top four j-bits,
+// so the first radix-16 round (stages M-1..M-4) runs on the loaded registers
+// with no LDS exchange; the second (stages K-1..0, K = M - 4) runs through
+// LDS as ntt_round does.  Every twiddle depends only on the lane and the
+// tile (15 for the first round, 2^K - 1 for the second; none on the
+// register group of the second round, whose column bits are the lane's):
+//   first:  w_{2^(s_lo+d+1)}^(((v << K | j0) << s_lo) | lo),  d = K + dd, v < 2^dd
+//   second: w_{2^(s_lo+dd+1)}^((v << s_lo) | lo),             v < 2^dd
+// with lo = lo_base + q, all loaded with the tile's words.  Per tile: loads |
+// round 1 in registers | write | barrier | round 2 | barrier | stores: 20%
+// fewer VALU instructions than ntt_round4 + ntt_round over LDS, at four
+// work-groups per CU (a persistent, prefetching form needs ~230 VGPRs for
+// the words in flight and the 30 twiddles, two waves per SIMD, and ran 9%
+// slower).
+
+template <int M>
+constexpr int strided2_nwb()
+{
+    return (1 << (M - 4)) - 1;
+}
+
+// offset of stage dd's values in a table of stages n-1..0 (2^dd values each)
+constexpr int stage_off(int n, int dd)
+{
+    int off = 0;
+    for (int d = n - 1; d > dd; d--) off += 1 << d;
+    return off;
+}
+
+template <int M>
+__device__ __forceinline__ void strided2_tw_load(const NttPass &a, uint64_t tile, uint32_t t, uint64_t (&wa)[15],
+                                                 uint64_t (&wb)[strided2_nwb<M>()])
+{
+    constexpr int K = M - 4, C = kNttTileLog - M;
+    const uint64_t lo = ((tile & ((1ull << (a.s_lo - C)) - 1)) << C) + (t & ((1u << C) - 1));
+    const uint64_t j0 = t >> C;
+    const cptr<const uint64_t *> tabs = cmem(a.tw_lg);
+#pragma unroll
+    for (int dd = 3; dd >= 0; dd--) {
+        const uint32_t lg = a.s_lo + (uint32_t)(K + dd) + 1;
+        const uint64_t *tab = tabs[lg];
+#pragma unroll
+        for (int v = 0; v < (1 << dd); v++)
+            wa[stage_off(4, dd) + v] = tw_at(tab, lg, ((((uint64_t)v << K) | j0) << a.s_lo) | lo, a.inverse);
+    }
+#pragma unroll
+    for (int dd = K - 1; dd >= 0; dd--) {
+        const uint32_t lg = a.s_lo + (uint32_t)dd + 1;
+        const uint64_t *tab = tabs[lg];
+#pragma unroll
+        for (int v = 0; v < (1 << dd); v++)
+            wb[stage_off(K, dd) + v] = tw_at(tab, lg, ((uint64_t)v << a.s_lo) | lo, a.inverse);
+    }
+}
+
+// First round: stages K+3..K on the registers (element i = top j-bits).
+__device__ __forceinline__ void strided2_round1(uint64_t (&x)[kNttPerThread], const uint64_t (&wa)[15])
+{
+#pragma unroll
+    for (int dd = 3; dd >= 0; dd--) {
+#pragma unroll
+        for (int e = 0; e < kNttPerThread; e++) {
+            if ((e >> dd) & 1) continue;
+            const int f = e | (1 << dd);
+            const uint64_t y0 = x[e], y1 = x[f];
+            x[e] = gl::add(y0, y1);
+            x[f] = gl::mul(gl::sub(y0, y1), wa[stage_off(4, dd) + (e & ((1 << dd) - 1))]);
+        }
+    }
+}
+
+// Second round: stages K-1..0 through LDS (ntt_round with b0 = 0, RB = c).
+template <int M>
+__device__ __forceinline__ void strided2_round2(uint64_t *lds, uint32_t t, const uint64_t (&wb)[strided2_nwb<M>()])
+{
+    constexpr int K = M - 4, C = kNttTileLog - M, E = kNttPerThread;
+    uint64_t x[E];
+    uint32_t sl[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const uint32_t rb = (uint32_t)e & ((1u << K) - 1);
+        const uint32_t other = t | (((uint32_t)e >> K) << 8);
+        sl[e] = lds_slot(((other >> C) << (C + K)) | (rb << C) | (other & ((1u << C) - 1)));
+        x[e] = lds[sl[e]];
+    }
+#pragma unroll
+    for (int dd = K - 1; dd >= 0; dd--) {
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            if ((e >> dd) & 1) continue;
+            const int f = e | (1 << dd);
+            const uint64_t y0 = x[e], y1 = x[f];
+            x[e] = gl::add(y0, y1);
+            x[f] = gl::mul(gl::sub(y0, y1), wb[stage_off(K, dd) + (e & ((1 << dd) - 1))]);
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < E; e++) lds[sl[e]] = x[e];
+}
+
+// tile_step of a strided pass with m = M >= 5 stages (c = 12 - M < 8)
+template <int M>
+__device__ __forceinline__ uint64_t strided2_step(const NttPass &a, int i)
+{
+    return (uint64_t)i << (M - 4 + a.s_lo);
+}
+
+// Rows of a two-round strided tile split into wave-uniform tile bases and
+// the lane's 32-bit offset ((j0 << s_lo) + q < 2^(L-4) + 2^s_lo): element i
+// of lane t sits at base + (i << (K + s_lo)) + off.
+struct Strided2Rows {
+    const uint64_t *src, *fac;
+    uint64_t *buf;
+};
+
+template <int MODE, int M>
+__device__ __forceinline__ Strided2Rows strided2_rows(const NttPass &a, uint64_t tile)
+{
+    constexpr int C = kNttTileLog - M;
+    const uint32_t chunk_bits = a.s_lo - C;
+    const uint64_t g = tile >> chunk_bits, ch = tile & ((1ull << chunk_bits) - 1);
+    Strided2Rows r;
+    // group g of 2^(s_lo + M) elements (one block of 2^L in the first pass,
+    // where an LDE pass reads: s_lo + M = L)
+    r.buf = a.buf + (g << (a.s_lo + M)) + (ch << C);
+    if constexpr (MODE == kSrcLde) {
+        r.src = a.src + ((g >> a.rate) << a.L) + (ch << C);
+        r.fac = a.coset + ((g & ((1ull << a.rate) - 1)) << a.L) + (ch << C);
+    } else {
+        r.src = r.buf;
+        r.fac = nullptr;
+    }
+    return r;
+}
+
+template <int M>
+__device__ __forceinline__ uint32_t strided2_off(const NttPass &a, uint32_t t)
+{
+    constexpr int C = kNttTileLog - M;
+    return ((t >> C) << a.s_lo) + (t & ((1u << C) - 1));
+}
+
+// Work-group k runs tile first + k of the list (passes of one source mode
+// and M stages; grid = the tiles of the range).
+template <int MODE, int M>
+__global__ void __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_eu(4)))
+k_ntt_strided2_tiles(const NttPass *__restrict__ passes, uint32_t npasses, NttPass one, uint64_t first)
 {
     __shared__ uint64_t lds[(1 << kNttTileLog) + (1 << kNttTileLog) / 32];
     const uint32_t t = threadIdx.x;
     const uint64_t b = first + blockIdx.x;
-    if (b >= ntiles) return;
-    NttPass a = pass_at(passes, npasses, one, b);
-    uint64_t tile = b - a.tile_begin;
-    uint64_t w[strided_ntw<K>()];
-    strided_tw_load<K>(a, tile, t, w);
-    uint64_t raw[kNttPerThread];
-    strided_fetch<MODE, K>(a, tile, t, raw);
-    {
-        const StridedRows r = strided_rows<MODE>(a, tile, t);
+    const NttPass a = pass_at(passes, npasses, one, b);
+    const uint64_t tile = b - a.tile_begin;
+    uint64_t wa[15], wb[strided2_nwb<M>()];
+    strided2_tw_load<M>(a, tile, t, wa, wb);
+    uint64_t x[kNttPerThread];
+    const Strided2Rows r = strided2_rows<MODE, M>(a, tile);
+    const uint32_t off = strided2_off<M>(a, t);
 #pragma unroll
-        for (int i = 0; i < kNttPerThread; i++) {
-            uint64_t v = raw[i];
-            if constexpr (MODE == kSrcLde) v = gl::mul(v, r.fac[strided_step<K>(a, i)]);
-            lds[lds_slot(t + kNttThreads * i)] = v;
-        }
+    for (int i = 0; i < kNttPerThread; i++) x[i] = gmem(r.src + strided2_step<M>(a, i))[off];
+    if constexpr (MODE == kSrcLde) {
+#pragma unroll
+        for (int i = 0; i < kNttPerThread; i++) x[i] = gl::mul(x[i], gmem(r.fac + strided2_step<M>(a, i))[off]);
     }
-    uint64_t bn = b + gridDim.x;
-    NttPass an = a;
-    uint64_t tn = 0;
-    if (bn < ntiles) {
-        an = pass_at(passes, npasses, one, bn);
-        tn = bn - an.tile_begin;
-        strided_fetch<MODE, K>(an, tn, t, raw);
-    }
+    strided2_round1(x, wa);
+#pragma unroll
+    for (int i = 0; i < kNttPerThread; i++) lds[lds_slot(t + kNttThreads * i)] = x[i];
     __syncthreads();
-    while (true) {
-        strided_round<K>(lds, t, a.c, w);
-        __syncthreads();
-        const StridedRows ro = strided_rows<MODE>(a, tile, t);
-        if (bn >= ntiles) {
+    strided2_round2<M>(lds, t, wb);
+    __syncthreads();
 #pragma unroll
-            for (int i = 0; i < kNttPerThread; i++) ro.buf[strided_step<K>(a, i)] = lds[lds_slot(t + kNttThreads * i)];
-            break;
-        }
-        const uint64_t bn2 = bn + gridDim.x;
-        NttPass an2 = an;
-        uint64_t tn2 = 0;
-        if (bn2 < ntiles) {
-            an2 = pass_at(passes, npasses, one, bn2);
-            tn2 = bn2 - an2.tile_begin;
-        }
-        // the next tile's twiddles and factors, then this tile's stores
-        strided_tw_load<K>(an, tn, t, w);
-        const StridedRows rn = strided_rows<MODE>(an, tn, t);
-        uint64_t f[kNttPerThread];
-        if constexpr (MODE == kSrcLde) {
-#pragma unroll
-            for (int i = 0; i < kNttPerThread; i++) f[i] = rn.fac[strided_step<K>(an, i)];
-        }
-        // each lane reads and rewrites only its own slots
-#pragma unroll
-        for (int i = 0; i < kNttPerThread; i++) {
-            const uint32_t s = lds_slot(t + kNttThreads * i);
-            const uint64_t o = lds[s];
-            if constexpr (MODE == kSrcLde) lds[s] = gl::mul(raw[i], f[i]);
-            else lds[s] = raw[i];
-            ro.buf[strided_step<K>(a, i)] = o;
-        }
-        if (bn2 < ntiles) strided_fetch<MODE, K>(an2, tn2, t, raw);
-        a = an;
-        tile = tn;
-        bn = bn2;
-        an = an2;
-        tn = tn2;
-        __syncthreads();
-    }
+    for (int i = 0; i < kNttPerThread; i++)
+        gmem(r.buf + strided2_step<M>(a, i))[off] = lds[lds_slot(t + kNttThreads * i)];
 }
 
 // In-place bit reversal of every column of length 2^lg (lg >= 2*kBrTile),
