@@ -431,9 +431,33 @@ __device__ __forceinline__ void contig_fetch(const NttPass &a, uint64_t tile, ui
     for (int i = 0; i < kNttPerThread; i++) raw[i] = src[i << 8];
 }
 
+// Stages m-1..8 of a contiguous pass (m > 8) on lane t's words t + 256 i of
+// the tile, in registers: in the load layout element i differs from its
+// partner of stage 8 + dd in bit dd of i, and its twiddle w_{2^(9+dd)}^(j mod
+// 2^(8+dd)), j mod 2^(8+dd) = (i mod 2^dd) << 8 | t, is w_4096^(that << (3 - dd))
+// from the work-group's LDS table.  The rounds left start at stage 7.
+__device__ __forceinline__ void contig_reg_round(uint64_t (&x)[kNttPerThread], uint32_t m, uint32_t t,
+                                                 const uint64_t *twl)
+{
+#pragma unroll
+    for (int dd = 3; dd >= 0; dd--) {
+        if ((uint32_t)dd + 8 >= m) continue;
+#pragma unroll
+        for (int e = 0; e < kNttPerThread; e++) {
+            if ((e >> dd) & 1) continue;
+            const int f = e | (1 << dd);
+            const uint64_t w = twl[((((uint32_t)e & ((1u << dd) - 1)) << 8) | t) << (3 - dd)];
+            const uint64_t y0 = x[e], y1 = x[f];
+            x[e] = gl::add(y0, y1);
+            x[f] = gl::mul(gl::sub(y0, y1), w);
+        }
+    }
+}
+
+// The stages below 8 (all of them when m <= 8) through LDS.
 __device__ __forceinline__ void contig_rounds(uint64_t *lds, const NttPass &a, uint32_t t, const uint64_t *twl)
 {
-    uint32_t rem = a.m;
+    uint32_t rem = a.m < 8 ? a.m : 8u;
     while (rem >= 4) {
         rem -= 4;
         if (rem == 0) ntt_round_last<4>(lds, t, a.inverse);
@@ -462,6 +486,8 @@ k_ntt_contig_pipe(const NttPass *__restrict__ passes, uint32_t npasses, NttPass 
     load_twl(a, t, twl);
     uint64_t raw[kNttPerThread];
     contig_fetch(a, tile, t, raw);
+    __syncthreads();  // twl complete
+    contig_reg_round(raw, a.m, t, twl);
 #pragma unroll
     for (int i = 0; i < kNttPerThread; i++) lds[lds_slot(t + kNttThreads * i)] = raw[i];
     uint64_t bn = b + gridDim.x;
@@ -492,15 +518,16 @@ k_ntt_contig_pipe(const NttPass *__restrict__ passes, uint32_t npasses, NttPass 
         if (an.inverse != dir) {
             load_twl(an, t, twl);  // every lane is past the rounds that read twl
             dir = an.inverse;
+            __syncthreads();  // and every lane's part of the new table is written
         }
-        // each lane reads and rewrites only its own slots
+        // each lane reads and rewrites only its own slots: this tile's
+        // results out, then the next tile's top stages in registers and its
+        // words in
 #pragma unroll
-        for (int i = 0; i < kNttPerThread; i++) {
-            const uint32_t sl = lds_slot(t + kNttThreads * i);
-            const uint64_t o = lds[sl];
-            lds[sl] = raw[i];
-            out[i << 8] = o;
-        }
+        for (int i = 0; i < kNttPerThread; i++) out[i << 8] = lds[lds_slot(t + kNttThreads * i)];
+        contig_reg_round(raw, an.m, t, twl);
+#pragma unroll
+        for (int i = 0; i < kNttPerThread; i++) lds[lds_slot(t + kNttThreads * i)] = raw[i];
         if (bn2 < ntiles) contig_fetch(an2, tn2, t, raw);
         a = an;
         tile = tn;
