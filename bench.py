@@ -64,12 +64,13 @@ LEAF_TRAFFIC_BYTES_PER_COMMIT = {(8, 0): 8.175e8, (9, 0): 8.089e8, (10, 0): 8.11
 LEAF_KERNEL_SOURCES = ("gl64.h", "gl_constants.h", "gmem.h", "hash.h", "poseidon_mfma.h", "merkle_kernels.h",
                        "merkle_launch.h", "merkle_launch_impl.h", "merkle_types.h", "merkle_poseidon.hip")
 #   NTT passes (both), same scaling
-NTT_TRAFFIC_BYTES_PER_COMMIT = 8.853e8
+NTT_TRAFFIC_BYTES_PER_COMMIT = 8.855e8
 #   VALU wave-instructions of the NTT launches per commit (both passes,
 #   SQ_INSTS_VALU), keyed by the LDE's pass split (stages per pass), measured
 #   on the sources NTT_COUNTERS_SOURCES
-NTT_VALU_WAVE_INSTS_PER_COMMIT = {(3, 12): 1.4266e8, (4, 11): 1.445e8}
-NTT_COUNTERS_SOURCES = "3a4d9b1caba6e601"
+NTT_VALU_WAVE_INSTS_PER_COMMIT = {(3, 12): 1.413e8, (4, 11): 1.415e8, (5, 10): 1.519e8, (6, 9): 1.462e8,
+                                  (7, 8): 1.511e8, (8, 7): 1.491e8}
+NTT_COUNTERS_SOURCES = "5ab03778ad29582e"
 # 32-bit VALU issue peak: 256 CUs x 4 SIMDs x one wave64 instruction per 4
 # cycles x 2.4 GHz = 39.3 T lane-ops/s (MI355X's published 78.6 TFLOP/s
 # vector FP64 counts an FMA as 2 flops on the same issue rate)
