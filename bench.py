@@ -68,9 +68,9 @@ NTT_TRAFFIC_BYTES_PER_COMMIT = 8.855e8
 #   VALU wave-instructions of the NTT launches per commit (both passes,
 #   SQ_INSTS_VALU), keyed by the LDE's pass split (stages per pass), measured
 #   on the sources NTT_COUNTERS_SOURCES
-NTT_VALU_WAVE_INSTS_PER_COMMIT = {(3, 12): 1.413e8, (4, 11): 1.415e8, (5, 10): 1.519e8, (6, 9): 1.462e8,
-                                  (7, 8): 1.511e8, (8, 7): 1.491e8}
-NTT_COUNTERS_SOURCES = "5ab03778ad29582e"
+NTT_VALU_WAVE_INSTS_PER_COMMIT = {(3, 12): 1.478e8, (4, 11): 1.463e8, (5, 10): 1.560e8, (6, 9): 1.500e8,
+                                  (7, 8): 1.547e8, (8, 7): 1.530e8}
+NTT_COUNTERS_SOURCES = "5cd02a58bc4df9c8"
 # 32-bit VALU issue peak: 256 CUs x 4 SIMDs x one wave64 instruction per 4
 # cycles x 2.4 GHz = 39.3 T lane-ops/s (MI355X's published 78.6 TFLOP/s
 # vector FP64 counts an FMA as 2 flops on the same issue rate)

@@ -93,7 +93,7 @@ private:
         uint64_t blocks;
         int kind;           // 0 NTT, 1 Merkle
         int hash;           // NTT: contiguous; Merkle: hash type
-        int pipe;           // NTT: ntt_pipe_kind of the list
+        int kernel;         // NTT: ntt_kernel_kind of the list
         uint32_t tile_log;  // Merkle work-group size (log2 lanes)
     };
     void issue(hipStream_t s, const std::vector<Launch> &launches, const LaunchPieces &pieces);

@@ -194,16 +194,16 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
         ZKX_HIP(hipMemsetAsync(ctr_.p, 0, ctr_.bytes, stream_));
         ctr_dirty_ = false;
     }
-    // NTT pass k of every job, one launch per (contiguity, pipelined kernel
-    // kind): jobs whose passes the pipelined kernels take (ntt_pipe_kind of
+    // NTT pass k of every job, one launch per (contiguity, kernel kind):
+    // jobs whose passes the register-round kernels take (ntt_kernel_kind of
     // the single pass) are not held back by others in the same launch (a
-    // launch runs pipelined only if all its passes qualify alike)
+    // launch uses such a kernel only if all its passes qualify alike)
     for (size_t k = 0; k < max_ntt; k++) {
         std::map<std::pair<int, int>, std::vector<size_t>> groups;
         for (size_t i = 0; i < n; i++) {
             if (k >= ntt_plan[i].size()) continue;
             const NttPass &p = ntt_plan[i][k];
-            groups[{p.s_lo == 0 ? 1 : 0, ntt_pipe_kind(&p, 1)}].push_back(i);
+            groups[{p.s_lo == 0 ? 1 : 0, ntt_kernel_kind(&p, 1)}].push_back(i);
         }
         for (const auto &grp : groups) {
             std::vector<NttPass> list;
@@ -256,7 +256,7 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
             key.push_back(pieces.merkle_groups);
             for (const Launch &l : launches)
                 key.insert(key.end(), {(uint64_t)l.offset, l.count, l.blocks, (uint64_t)l.kind, (uint64_t)l.hash,
-                                       (uint64_t)l.pipe, l.tile_log});
+                                       (uint64_t)l.kernel, l.tile_log});
             auto it = graphs_.find(key);
             if (it == graphs_.end() && seen_.size() >= 64 * kMaxGraphs) seen_.clear();
             if (it == graphs_.end() && ++seen_[key] >= kCaptureAfter) {
@@ -305,7 +305,7 @@ void Engine::issue(hipStream_t s, const std::vector<Launch> &launches, const Lau
     for (const Launch &l : launches) {
         const char *base = static_cast<const char *>(desc_.p) + l.offset;
         if (l.kind == 0) {
-            launch_ntt_list(s, reinterpret_cast<const NttPass *>(base), l.count, l.blocks, l.hash == 1, l.pipe,
+            launch_ntt_list(s, reinterpret_cast<const NttPass *>(base), l.count, l.blocks, l.hash == 1, l.kernel,
                             "batch_ntt", pieces.ntt_tiles);
         } else {
             launch_merkle_pass(s, l.hash, reinterpret_cast<const MerkleArgs *>(base), l.count, MerkleArgs{},

@@ -41,21 +41,22 @@ uint64_t plan_dif(std::vector<NttPass> &out, uint64_t *buf, uint64_t nblk, uint3
 // stages first (the last pass contiguous).
 std::vector<uint32_t> ntt_pass_sizes(uint32_t L, const Tuning &tun);
 
-// Which persistent, pipelined kernel a launch over these passes can use:
-// 0 none; 1 contiguous (every pass in place and a whole number of tiles);
+// Which register-round kernel a launch over these passes can use (each runs
+// one tile per work-group): 0 none (k_ntt_tiles); 1 contiguous (every pass
+// in place and a whole number of tiles; stages above 8 in registers);
 // 2 + 4 * lde + (m - 1) strided single-round (every pass strided with the
-// same m <= 4 stages and the same source, in place or LDE, persistent and
-// pipelined); 10 + 4 * lde + (m - 5) strided two-round (the same with
-// 5 <= m <= 8, one tile per work-group, first round in registers).
-int ntt_pipe_kind(const NttPass *host_list, size_t n);
+// same m <= 4 stages and the same source, in place or LDE; registers only);
+// 10 + 4 * lde + (m - 5) strided two-round (the same with 5 <= m <= 8, the
+// first round in registers).
+int ntt_kernel_kind(const NttPass *host_list, size_t n);
 
 // One launch over n pass descriptors in device memory (sorted by
-// tile_begin), all contiguous (s_lo == 0) or all strided; `pipe_kind` as
-// ntt_pipe_kind of the same list.  chunk > 0: launches over ranges of at
+// tile_begin), all contiguous (s_lo == 0) or all strided; `kernel_kind` as
+// ntt_kernel_kind of the same list.  chunk > 0: launches over ranges of at
 // most `chunk` tiles instead, in order (other streams' kernels can be
 // dispatched between them).
 void launch_ntt_list(hipStream_t stream, const NttPass *dev_list, uint32_t n, uint64_t blocks, bool contiguous,
-                     int pipe_kind, const char *name, uint64_t chunk = 0);
+                     int kernel_kind, const char *name, uint64_t chunk = 0);
 
 // table[i] = base^i, i < n.
 void fill_powers(uint64_t *dst, uint64_t n, uint64_t base, hipStream_t s);

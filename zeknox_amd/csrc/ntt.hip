@@ -4,8 +4,6 @@
 #include "ntt.h"
 
 #include <algorithm>
-#include <map>
-#include <mutex>
 #include <string>
 #include <vector>
 
@@ -14,36 +12,6 @@
 namespace zkx {
 
 namespace {
-
-// Work-groups of a persistent kernel resident at once on the current device
-// (its grid), cached per device and kernel.
-template <class Kernel>
-uint64_t resident_groups(Kernel kernel)
-{
-    static std::mutex mu;
-    static std::map<std::pair<int, const void *>, uint64_t> cache;
-    int dev = 0;
-    ZKX_HIP(hipGetDevice(&dev));
-    const auto key = std::make_pair(dev, reinterpret_cast<const void *>(kernel));
-    std::lock_guard<std::mutex> g(mu);
-    auto it = cache.find(key);
-    if (it != cache.end()) return it->second;
-    int per_cu = 0, cus = 0;
-    ZKX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kNttThreads, 0));
-    ZKX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const uint64_t n = (uint64_t)std::max(per_cu, 1) * (uint64_t)std::max(cus, 1);
-    cache[key] = n;
-    return n;
-}
-
-// A persistent launch over the tiles [first, end) of the list.
-template <class Kernel>
-void launch_persistent(Kernel kernel, hipStream_t stream, const NttPass *dev_list, uint32_t n, const NttPass &one,
-                       uint64_t first, uint64_t end)
-{
-    const uint64_t groups = std::min(end - first, resident_groups(kernel));
-    hipLaunchKernelGGL(kernel, dim3((uint32_t)groups), dim3(kNttThreads), 0, stream, dev_list, n, one, end, first);
-}
 
 // One work-group per tile of [first, end).
 template <int MODE>
@@ -74,12 +42,13 @@ void launch_strided2(int m, hipStream_t stream, const NttPass *dev_list, uint32_
 }
 
 // A launch over the tiles [first, end) of the passes (device list, or `one`
-// when n == 0) with the kernel ntt_pipe_kind chose; false for kind 0.
-bool launch_pipe(int kind, hipStream_t stream, const NttPass *dev_list, uint32_t n, const NttPass &one,
+// when n == 0) with the kernel ntt_kernel_kind chose; false for kind 0.
+bool launch_kind(int kind, hipStream_t stream, const NttPass *dev_list, uint32_t n, const NttPass &one,
                  uint64_t first, uint64_t end)
 {
     if (kind == 1) {
-        launch_persistent(k_ntt_contig_pipe, stream, dev_list, n, one, first, end);
+        hipLaunchKernelGGL(k_ntt_contig_reg, dim3((uint32_t)(end - first)), dim3(kNttThreads), 0, stream, dev_list, n,
+                           one, first);
         return true;
     }
     if (kind >= 2 && kind < 10) {
@@ -102,7 +71,7 @@ void launch_pass(Device &d, const NttPass &p, uint64_t tiles, const char *name)
 {
     if (tiles > 0x7FFFFFFFull) invalid("transform too large for one launch");
     ProfScope ps(name, d.stream);
-    if (launch_pipe(ntt_pipe_kind(&p, 1), d.stream, nullptr, 0u, p, 0, tiles)) {
+    if (launch_kind(ntt_kernel_kind(&p, 1), d.stream, nullptr, 0u, p, 0, tiles)) {
     } else if (p.s_lo == 0)
         hipLaunchKernelGGL(k_ntt_tiles<true>, dim3((uint32_t)tiles), dim3(kNttThreads), 0, d.stream,
                            (const NttPass *)nullptr, 0u, p, (uint64_t)0);
@@ -127,7 +96,7 @@ void run_dif(Device &d, uint64_t *buf, uint64_t nblk, uint32_t L, uint32_t lg_tw
 
 }  // namespace
 
-int ntt_pipe_kind(const NttPass *host_list, size_t n)
+int ntt_kernel_kind(const NttPass *host_list, size_t n)
 {
     if (n == 0) return 0;
     const NttPass &p0 = host_list[0];
@@ -148,7 +117,7 @@ int ntt_pipe_kind(const NttPass *host_list, size_t n)
 }
 
 void launch_ntt_list(hipStream_t stream, const NttPass *dev_list, uint32_t n, uint64_t blocks, bool contiguous,
-                     int pipe_kind, const char *name, uint64_t chunk)
+                     int kernel_kind, const char *name, uint64_t chunk)
 {
     if (blocks == 0) return;
     if (blocks > 0x7FFFFFFFull) invalid("transform batch too large for one launch");
@@ -156,7 +125,7 @@ void launch_ntt_list(hipStream_t stream, const NttPass *dev_list, uint32_t n, ui
     const uint64_t step = chunk ? chunk : blocks;
     for (uint64_t first = 0; first < blocks; first += step) {
         const uint64_t end = std::min(blocks, first + step);
-        if (launch_pipe(pipe_kind, stream, dev_list, n, NttPass{}, first, end)) {
+        if (launch_kind(kernel_kind, stream, dev_list, n, NttPass{}, first, end)) {
         } else if (contiguous)
             hipLaunchKernelGGL(k_ntt_tiles<true>, dim3((uint32_t)(end - first)), dim3(kNttThreads), 0, stream,
                                dev_list, n, NttPass{}, first);

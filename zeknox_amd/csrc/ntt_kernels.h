@@ -42,15 +42,6 @@ __device__ __forceinline__ uint64_t tw12_at(const NttPass &a, uint32_t k)
     return gmem(a.tw12)[a.inverse ? ((0u - k) & ((1u << kNttTileLog) - 1)) : k];
 }
 
-// The same twiddle, k < 2048, from the work-group's LDS copy of the pass's
-// direction (TWL) or from the global table.
-template <bool TWL>
-__device__ __forceinline__ uint64_t tw12_get(const NttPass &a, const uint64_t *twl, uint32_t k)
-{
-    if constexpr (TWL) return twl[k];
-    else return tw12_at(a, k);
-}
-
 // Twiddle table of a strided pass's stage d (butterfly span 2^(s_lo+d+1)):
 // w_{2^(s_lo+d+1)}^k sits at index k, so the 2^c consecutive columns of a
 // tile read consecutive words (rather than words 2^(lg_tw-s_lo-d-1) apart
@@ -102,9 +93,9 @@ __device__ __forceinline__ uint64_t tile_step(const NttPass &a, int i)
 // 2^(4-K) groups taken from the extra element bits; the other u-bits come
 // from the lane id.  The twiddle of stage d for element e depends only on
 // (j mod 2^d) << s_lo | lo.
-template <bool CONTIG, int K, bool TWL = false>
+template <bool CONTIG, int K>
 __device__ __forceinline__ void ntt_round(uint64_t *lds, const NttPass &a, uint32_t t, uint32_t b0, uint32_t jpos,
-                                          uint64_t lo_base, const uint64_t *twl = nullptr)
+                                          uint64_t lo_base)
 {
     constexpr int E = kNttPerThread;
     const uint32_t RB = jpos + b0;
@@ -131,7 +122,7 @@ __device__ __forceinline__ void ntt_round(uint64_t *lds, const NttPass &a, uint3
             const uint32_t j = !CONTIG ? (u >> a.c) : (u & ((1u << a.m) - 1));
             uint64_t w;
             if constexpr (CONTIG) {
-                w = tw12_get<TWL>(a, twl, (j & ((1u << d) - 1)) << (kNttTileLog - 1 - d));
+                w = tw12_at(a, (j & ((1u << d) - 1)) << (kNttTileLog - 1 - d));
             } else {
                 const uint64_t lo_part = lo_base + (u & ((1u << a.c) - 1));
                 const uint64_t ex = (((uint64_t)(j & ((1u << d) - 1))) << a.s_lo) | lo_part;
@@ -148,9 +139,9 @@ __device__ __forceinline__ void ntt_round(uint64_t *lds, const NttPass &a, uint3
 
 // Full-width round (K = 4, no extra element bits): the lane's other u-bits
 // are its id, so the 15 distinct twiddles of the round are loaded once.
-template <bool CONTIG, bool TWL = false>
+template <bool CONTIG>
 __device__ __forceinline__ void ntt_round4(uint64_t *lds, const NttPass &a, uint32_t t, uint32_t b0, uint32_t jpos,
-                                           uint64_t lo_base, const uint64_t *twl = nullptr)
+                                           uint64_t lo_base)
 {
     constexpr int E = kNttPerThread;
     const uint32_t RB = jpos + b0;
@@ -171,7 +162,7 @@ __device__ __forceinline__ void ntt_round4(uint64_t *lds, const NttPass &a, uint
             // contiguous pass: w_{2^(d+1)}^x = w_4096^(x << (11 - d))
 #pragma unroll
             for (int v = 0; v < (1 << dd); v++)
-                w[v] = tw12_get<TWL>(a, twl, (j0 | ((uint32_t)v << b0)) << (kNttTileLog - 1 - d));
+                w[v] = tw12_at(a, (j0 | ((uint32_t)v << b0)) << (kNttTileLog - 1 - d));
         } else {
             const uint64_t *tab = stage_table(a, lg);
 #pragma unroll
@@ -367,27 +358,10 @@ k_ntt_tiles(const NttPass *__restrict__ passes, uint32_t npasses, NttPass one, u
     }
 }
 
-// ---- contiguous passes, persistent and pipelined
+// ---- descriptors of the one-tile-per-work-group kernels
 //
-// A contiguous pass is load -> up to 12 stages in LDS -> store per tile; a
-// work-group that runs one tile and exits leaves HBM idle while it computes.
-// Here each work-group walks tiles first + blockIdx.x, +gridDim.x, ... (up to
-// ntiles: a launch covers the tiles [first, ntiles) of its list) and issues the
-// global loads of its next tile before the rounds of the current one, so the
-// loads (16 per lane, in registers) are in flight while it computes.  The
-// stage twiddles come from an LDS copy of the w_4096 table, loaded once per
-// work-group, so the rounds issue no global load: vector-memory counters
-// retire in order, and a wait for a load issued in the rounds would also wait
-// for the prefetch.  Per tile:
-//   rounds(k) | per position: read result(k), write input(k+1), store result(k)
-//   | prefetch(k+2) | barrier
-// The kernel takes in-place passes over whole tiles only (ntt_pipe_ok):
-// input transforms and ragged ends would put branches between the loads and
-// stores, and across a branch the compiler loses its count of the memory
-// operations in flight and waits for all of them.
-
-constexpr int kNttPipeWaves = 3;  // per SIMD: 49 KB of LDS and 168 VGPRs per work-group
-constexpr int kTw12Half = 1 << (kNttTileLog - 1);
+// A launch's descriptor list is read through the constant address space
+// (scalar loads into SGPRs; as vector loads they cost 32 VGPRs).
 
 __device__ __forceinline__ NttPass pass_at(const NttPass *passes, uint32_t npasses, const NttPass &one, uint64_t b)
 {
@@ -410,132 +384,65 @@ __device__ __forceinline__ NttPass pass_at(const NttPass *passes, uint32_t npass
     return r;
 }
 
-// The LDS copy of the contiguous stage twiddles w_4096^k, k < 2048, in the
-// pass's direction: all eight loads per lane in flight, then the stores.
-__device__ __forceinline__ void load_twl(const NttPass &a, uint32_t t, uint64_t *twl)
+// ---- contiguous in-place passes over whole tiles, one tile per work-group
+//
+// Lane t loads the tile's words t + 256 i, which are, for the stages 8..m-1
+// of the pass (m > 8), the butterfly groups of those stages: element i
+// differs from its partner of stage 8 + dd in bit dd of i, and its twiddle
+// w_{2^(9+dd)}^(j mod 2^(8+dd)), j mod 2^(8+dd) = (i mod 2^dd) << 8 | t, is
+// w_4096^(that << (3 - dd)) from the global table (16 KB of it in use).  So
+// those stages run on the loaded registers and only the stages below 8 go
+// through LDS (two register rounds, the last with its small roots as
+// shifts).  Four work-groups per CU at 127 VGPRs.  This replaced a
+// persistent kernel that prefetched the next tile and staged the stage
+// twiddles in LDS (three waves per SIMD, 164-168 VGPRs): batched LDE NTT
+// 5.13 -> 4.75 ms per 16 commits, the plain 12-stage pass 0.216 -> 0.198 ms,
+// same GPU, alternating builds.  The kernel takes in-place passes over whole
+// tiles only (ntt_kernel_kind 1); ragged, coset and LDE-source contiguous passes
+// run on k_ntt_tiles.
+__global__ void __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_eu(4)))
+k_ntt_contig_reg(const NttPass *__restrict__ passes, uint32_t npasses, NttPass one, uint64_t first)
 {
-    constexpr int R = kTw12Half / kNttThreads;
-    uint64_t v[R];
+    __shared__ uint64_t lds[(1 << kNttTileLog) + (1 << kNttTileLog) / 32];
+    const uint32_t t = threadIdx.x;
+    const uint64_t b = first + blockIdx.x;
+    const NttPass a = pass_at(passes, npasses, one, b);
+    const uint64_t tile = b - a.tile_begin;
+    const gptr<uint64_t> p = gmem(a.buf) + (tile << kNttTileLog) + t;
+    uint64_t x[kNttPerThread];
 #pragma unroll
-    for (int r = 0; r < R; r++) v[r] = tw12_at(a, t + kNttThreads * r);
-#pragma unroll
-    for (int r = 0; r < R; r++) twl[t + kNttThreads * r] = v[r];
-}
-
-// Words t + 256 i of the tile.
-__device__ __forceinline__ void contig_fetch(const NttPass &a, uint64_t tile, uint32_t t,
-                                             uint64_t (&raw)[kNttPerThread])
-{
-    const gptr<const uint64_t> src = gmem((const uint64_t *)a.buf) + (tile << kNttTileLog) + t;
-#pragma unroll
-    for (int i = 0; i < kNttPerThread; i++) raw[i] = src[i << 8];
-}
-
-// Stages m-1..8 of a contiguous pass (m > 8) on lane t's words t + 256 i of
-// the tile, in registers: in the load layout element i differs from its
-// partner of stage 8 + dd in bit dd of i, and its twiddle w_{2^(9+dd)}^(j mod
-// 2^(8+dd)), j mod 2^(8+dd) = (i mod 2^dd) << 8 | t, is w_4096^(that << (3 - dd))
-// from the work-group's LDS table.  The rounds left start at stage 7.
-__device__ __forceinline__ void contig_reg_round(uint64_t (&x)[kNttPerThread], uint32_t m, uint32_t t,
-                                                 const uint64_t *twl)
-{
+    for (int i = 0; i < kNttPerThread; i++) x[i] = p[i << 8];
 #pragma unroll
     for (int dd = 3; dd >= 0; dd--) {
-        if ((uint32_t)dd + 8 >= m) continue;
+        if ((uint32_t)dd + 8 >= a.m) continue;
+        uint64_t w[8];
+#pragma unroll
+        for (int v = 0; v < (1 << dd); v++) w[v] = tw12_at(a, (((uint32_t)v << 8) | t) << (3 - dd));
 #pragma unroll
         for (int e = 0; e < kNttPerThread; e++) {
             if ((e >> dd) & 1) continue;
             const int f = e | (1 << dd);
-            const uint64_t w = twl[((((uint32_t)e & ((1u << dd) - 1)) << 8) | t) << (3 - dd)];
             const uint64_t y0 = x[e], y1 = x[f];
             x[e] = gl::add(y0, y1);
-            x[f] = gl::mul(gl::sub(y0, y1), w);
+            x[f] = gl::mul(gl::sub(y0, y1), w[e & ((1 << dd) - 1)]);
         }
     }
-}
-
-// The stages below 8 (all of them when m <= 8) through LDS.
-__device__ __forceinline__ void contig_rounds(uint64_t *lds, const NttPass &a, uint32_t t, const uint64_t *twl)
-{
+#pragma unroll
+    for (int i = 0; i < kNttPerThread; i++) lds[lds_slot(t + kNttThreads * i)] = x[i];
+    __syncthreads();
     uint32_t rem = a.m < 8 ? a.m : 8u;
     while (rem >= 4) {
         rem -= 4;
         if (rem == 0) ntt_round_last<4>(lds, t, a.inverse);
-        else ntt_round4<true, true>(lds, a, t, rem, 0, 0, twl);
+        else ntt_round4<true>(lds, a, t, rem, 0, 0);
         __syncthreads();
     }
     if (rem == 3) ntt_round_last<3>(lds, t, a.inverse);
     else if (rem == 2) ntt_round_last<2>(lds, t, a.inverse);
     else if (rem == 1) ntt_round_last<1>(lds, t, a.inverse);
     if (rem) __syncthreads();
-}
-
-// Launch with at most ntiles - first work-groups over passes that satisfy
-// ntt_pipe_ok.
-__global__ void __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_eu(kNttPipeWaves)))
-k_ntt_contig_pipe(const NttPass *__restrict__ passes, uint32_t npasses, NttPass one, uint64_t ntiles, uint64_t first)
-{
-    __shared__ uint64_t lds[(1 << kNttTileLog) + (1 << kNttTileLog) / 32];
-    __shared__ uint64_t twl[kTw12Half];
-    const uint32_t t = threadIdx.x;
-    const uint64_t b = first + blockIdx.x;
-    if (b >= ntiles) return;
-    NttPass a = pass_at(passes, npasses, one, b);
-    uint64_t tile = b - a.tile_begin;
-    uint32_t dir = a.inverse;
-    load_twl(a, t, twl);
-    uint64_t raw[kNttPerThread];
-    contig_fetch(a, tile, t, raw);
-    __syncthreads();  // twl complete
-    contig_reg_round(raw, a.m, t, twl);
 #pragma unroll
-    for (int i = 0; i < kNttPerThread; i++) lds[lds_slot(t + kNttThreads * i)] = raw[i];
-    uint64_t bn = b + gridDim.x;
-    NttPass an = a;
-    uint64_t tn = 0;
-    if (bn < ntiles) {
-        an = pass_at(passes, npasses, one, bn);
-        tn = bn - an.tile_begin;
-        contig_fetch(an, tn, t, raw);
-    }
-    __syncthreads();
-    while (true) {
-        contig_rounds(lds, a, t, twl);  // ends with a barrier
-        const gptr<uint64_t> out = gmem(a.buf) + (tile << kNttTileLog) + t;
-        if (bn >= ntiles) {
-#pragma unroll
-            for (int i = 0; i < kNttPerThread; i++) out[i << 8] = lds[lds_slot(t + kNttThreads * i)];
-            break;
-        }
-        // the tile after next: its pass, looked up before this tile's stores
-        const uint64_t bn2 = bn + gridDim.x;
-        NttPass an2 = an;
-        uint64_t tn2 = 0;
-        if (bn2 < ntiles) {
-            an2 = pass_at(passes, npasses, one, bn2);
-            tn2 = bn2 - an2.tile_begin;
-        }
-        if (an.inverse != dir) {
-            load_twl(an, t, twl);  // every lane is past the rounds that read twl
-            dir = an.inverse;
-            __syncthreads();  // and every lane's part of the new table is written
-        }
-        // each lane reads and rewrites only its own slots: this tile's
-        // results out, then the next tile's top stages in registers and its
-        // words in
-#pragma unroll
-        for (int i = 0; i < kNttPerThread; i++) out[i << 8] = lds[lds_slot(t + kNttThreads * i)];
-        contig_reg_round(raw, an.m, t, twl);
-#pragma unroll
-        for (int i = 0; i < kNttPerThread; i++) lds[lds_slot(t + kNttThreads * i)] = raw[i];
-        if (bn2 < ntiles) contig_fetch(an2, tn2, t, raw);
-        a = an;
-        tile = tn;
-        bn = bn2;
-        an = an2;
-        tn = tn2;
-        __syncthreads();
-    }
+    for (int i = 0; i < kNttPerThread; i++) p[i << 8] = lds[lds_slot(t + kNttThreads * i)];
 }
 
 // ---- strided single-round passes, one tile per work-group, registers only
@@ -551,7 +458,7 @@ k_ntt_contig_pipe(const NttPass *__restrict__ passes, uint32_t npasses, NttPass 
 // SIMD and ran the 2^18 x 135 LDE pass 10% slower than this one at four or
 // five.  MODE is the pass's source (kSrcInPlace or kSrcLde), K its stage
 // count; a launch takes passes of one mode and one stage count
-// (ntt_pipe_kind).
+// (ntt_kernel_kind).
 
 // tile_step of a strided pass with m = K <= 4 stages (c = 12 - K >= 8): the
 // branches of the general form decided at compile time.
