@@ -119,8 +119,8 @@ RustError zkx_commit_batch(size_t device_id, const ZkxCommitJob *jobs, size_t nj
  * order on the same stream.  For callers that share the GPU with
  * latency-sensitive work on other streams: the device dispatches another
  * stream's kernels between two launches, not while a launch still has
- * work-groups to dispatch or its persistent work-groups hold every CU (the
- * commit server runs its large requests this way).  Same results. */
+ * work-groups to dispatch (the commit server runs its large requests this
+ * way).  Same results. */
 RustError zkx_commit_batch_pieces(size_t device_id, const ZkxCommitJob *jobs, size_t njobs, uint64_t ntt_tiles,
                                   uint64_t merkle_groups);
 /* Device working memory zkx_commit_batch needs for these jobs, in bytes.

@@ -10,7 +10,7 @@ import oracle_bind as ob
 
 pytestmark = pytest.mark.gpu
 
-# (cols, lg_n, rate, cap_height, hash): pipelined and one-tile NTT passes,
+# (cols, lg_n, rate, cap_height, hash): register-round and LDS-round NTT passes,
 # leaves of <= 4 elements, caps == leaves, four hash types
 SHAPES = [(2, 10, 1, 4, 0), (20, 11, 2, 4, 3), (80, 10, 3, 4, 0), (5, 3, 1, 0, 0), (17, 9, 2, 4, 1),
           (34, 8, 1, 3, 4), (135, 12, 1, 4, 0), (3, 7, 1, 2, 3)]

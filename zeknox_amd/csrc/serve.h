@@ -14,10 +14,10 @@
 //     on engines 1..nstreams-1: engine 0 only ever takes small batches, so
 //     a small request is never queued behind large ones for an engine; and
 //     a large batch's leaf launch (its longest kernel by far) is issued in
-//     pieces of kLargeMerkleChunk work-groups and its persistent NTT passes
-//     in ranges of kLargeNttChunk tiles: the device takes another stream's
-//     kernels between two of a stream's kernels, not while one still has
-//     work-groups to dispatch or holds every CU;
+//     pieces of kLargeMerkleChunk work-groups and its NTT passes in ranges
+//     of kLargeNttChunk tiles: the device takes another stream's kernels
+//     between two of a stream's kernels, not while one still has
+//     work-groups to dispatch;
 //   * the budget is one device allocation made when the server starts; a
 //     request is admitted only when its working set -- output buffers (leaf
 //     rows, digests, caps) followed by its LDE scratch -- can be placed in it
@@ -85,7 +85,7 @@ public:
     // 134 ms, mean latency 34 -> 24.5 ms, large requests' median service
     // 47 -> 50 ms; 2048 work-groups gave 114 ms small p99 but 56 ms large.
     static constexpr uint64_t kLargeMerkleChunk = 4096;
-    // Its persistent NTT passes likewise, in ranges of this many 4096-element
+    // Its NTT passes likewise, in ranges of this many 4096-element
     // tiles (2 GB of LDE per piece, 1-2 ms).
     static constexpr uint64_t kLargeNttChunk = 65536;
 
