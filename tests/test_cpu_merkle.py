@@ -96,6 +96,23 @@ def test_cpu_merkle_vs_reference_build(ht):
     assert np.array_equal(dig, rd)
 
 
+@pytest.mark.parametrize("ht", [0, 3])
+@pytest.mark.parametrize("lg_l,ls,h", [(12, 135, 4), (10, 9, 10), (11, 8, 2), (9, 5, 6), (8, 200, 0)])
+def test_cpu_merkle_vector_and_scalar_paths_agree(monkeypatch, ht, lg_l, ls, h):
+    """Poseidon and Poseidon2 hash eight leaves / nodes at a time with AVX-512
+    where the CPU has it; ZKX_CPU_NO_AVX512=1 forces the scalar path.  Both
+    give the same forest (and the oracle's)."""
+    leaves = ob.canonical(ls << lg_l, 31 * lg_l + ls + ht)
+    vec = cpu_tree(leaves, ls, h, ht)
+    monkeypatch.setenv("ZKX_CPU_NO_AVX512", "1")
+    sca = cpu_tree(leaves, ls, h, ht)
+    monkeypatch.delenv("ZKX_CPU_NO_AVX512")
+    assert all(np.array_equal(a, b) for a, b in zip(vec, sca))
+    if lg_l <= 10:
+        edig, ecap = ob.merkle(leaves, ls, h, ht)
+        assert np.array_equal(vec[1], ecap) and np.array_equal(vec[0], edig)
+
+
 def test_cpu_merkle_rejects_bad_sizes():
     leaves = np.zeros(8 * 4, np.uint64)
     with pytest.raises(zk.ZeknoxError):

@@ -10,6 +10,9 @@
 // together across a pool of std::threads (OMP_NUM_THREADS of them when set,
 // else one per hardware thread), then each tree level across all subtrees.
 //
+// Poseidon and Poseidon2 leaves and levels go eight at a time through the
+// AVX-512 forms in merkle_cpu_x8.hip where the CPU has them.
+//
 // Field arithmetic: canonical Goldilocks values, 128-bit products reduced
 // with 2^64 = 2^32 - 1 and 2^96 = -1 (mod p).  Poseidon runs in plonky2's
 // "fast partial round" form (the reference CPU's, poseidon.cpp:112-203): the
@@ -543,7 +546,17 @@ void fill_digests_linear(uint64_t *digests, uint64_t *caps, const uint64_t *leav
     if (leaf_size == 0) invalid("leaf size out of range");
     const Hasher H{hash_type, permutation_of(hash_type)};
     const uint64_t chunks = (leaf_size + 7) / 8;
+    // eight leaves or nodes per call where the hash has an AVX-512 form:
+    // groups of 8 aligned items, inside one subtree whenever the subtree
+    // width is a multiple of 8 (smaller widths run one at a time)
+    const bool x8 = have_x8(hash_type);
     if (nc == nl) {  // every leaf is its own cap
+        if (x8 && leaf_size > 4 && nl >= 8) {
+            parallel_for(nl / 8, 8 * chunks, [&](uint64_t g) {
+                hash_one_x8(hash_type, leaves + 8 * g * leaf_size, leaf_size, leaf_size, caps + 32 * g, 4);
+            });
+            return;
+        }
         parallel_for(nl, chunks, [&](uint64_t t) { H.one(leaves + t * leaf_size, leaf_size, caps + 4 * t); });
         return;
     }
@@ -555,11 +568,28 @@ void fill_digests_linear(uint64_t *digests, uint64_t *caps, const uint64_t *leav
     // digest i = heap node i + 2 (children 2i + 2, 2i + 3); the level of 2^j
     // nodes sits at [2^j - 2, 2^(j+1) - 2), the leaf hashes at [sl - 2, 2 sl - 2)
     const uint64_t sl = nl / nc, sdl = nd / nc;
-    parallel_for(nl, chunks, [&](uint64_t t) {
-        const uint64_t k = t / sl, i = t % sl;
-        H.one(leaves + t * leaf_size, leaf_size, digests + 4 * (k * sdl + sl - 2 + i));
-    });
+    if (x8 && leaf_size > 4 && sl % 8 == 0) {
+        parallel_for(nl / 8, 8 * chunks, [&](uint64_t g) {
+            const uint64_t t = 8 * g, k = t / sl, i = t % sl;
+            hash_one_x8(hash_type, leaves + t * leaf_size, leaf_size, leaf_size, digests + 4 * (k * sdl + sl - 2 + i),
+                        4);
+        });
+    } else {
+        parallel_for(nl, chunks, [&](uint64_t t) {
+            const uint64_t k = t / sl, i = t % sl;
+            H.one(leaves + t * leaf_size, leaf_size, digests + 4 * (k * sdl + sl - 2 + i));
+        });
+    }
     for (uint64_t w = sl / 2; w >= 2; w /= 2) {
+        if (x8 && w % 8 == 0) {
+            // nodes j..j+7 of a level: children at 8-word steps, outputs at 4
+            parallel_for(nc * w / 8, 8, [&](uint64_t g) {
+                const uint64_t q = 8 * g, k = q / w, j = w - 2 + q % w;
+                uint64_t *d = digests + 4 * k * sdl;
+                hash_two_x8(hash_type, d + 4 * (2 * j + 2), 8, d + 4 * j, 4);
+            });
+            continue;
+        }
         parallel_for(nc * w, 1, [&](uint64_t q) {
             const uint64_t k = q / w, j = w - 2 + q % w;
             uint64_t *d = digests + 4 * k * sdl;
