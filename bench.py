@@ -104,7 +104,8 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=200.0,
                     help="served-stream arrivals per second (default: about 80% of the highest rate the server "
                          "keeps up with, 240-260/s in the sweeps)")
-    ap.add_argument("--serve-streams", type=int, default=3, help="commit engines (streams) of the server")
+    ap.add_argument("--serve-streams", type=int, default=6,
+                    help="commit engines (streams) of the server (6: lowest latencies at 200/s, against 3 and 8)")
     ap.add_argument("--serve-trace", default=None,
                     help="write one CSV line per served request (times in ms from the first arrival)")
     ap.add_argument("--no-tune", action="store_true", help="batched path on the default schedule (no autotuning)")

@@ -75,7 +75,8 @@ def test_batch_config3_seeded_64_jobs(zk, golden):
         assert cs == e["caps_sha256"], f"job {j} caps"
 
 
-def test_served_config5_replay(zk, golden):
+@pytest.mark.parametrize("streams", [3, 6])
+def test_served_config5_replay(zk, golden, streams):
     spec = golden["served"]
     trace = served_trace(spec["seed"], spec["seconds"], spec["rate_hz"])
     exp = spec["requests"]
@@ -84,7 +85,8 @@ def test_served_config5_replay(zk, golden):
     pool.fill_random(spec["seed"])
     feet = [zk.Server.footprint(c, l, r, CAP_HEIGHT) for _, c, l, r, _ in trace]
     budget = int(max(feet) * 1.15)
-    srv = zk.Server(budget, streams=3, max_batch=16)
+    print(f"served replay with {streams} engines", flush=True)
+    srv = zk.Server(budget, streams=streams, max_batch=16)
     ids = {}
     lock = threading.Lock()
     errors = []

@@ -509,7 +509,7 @@ def merkle_plan(num_leaves, num_caps, tile_log):
 class Server:
     """Continuous-batching commit server (zeknox_ext.h zkx_server_*)."""
 
-    def __init__(self, budget_bytes, streams=2, max_batch=64, device_id=0):
+    def __init__(self, budget_bytes, streams=6, max_batch=64, device_id=0):
         h = ctypes.c_void_p()
         _check(lib.zkx_server_create(device_id, int(budget_bytes), streams, max_batch, ctypes.byref(h)))
         self.h = h.value
