@@ -96,7 +96,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--jobs", type=int, default=16, help="135 x 2^15 rate-3 commits per batched step")
+    ap.add_argument("--jobs", type=int, default=32, help="135 x 2^15 rate-3 commits per batched step")
     ap.add_argument("--no-graphs", action="store_true", help="batched steps launched kernel by kernel "
                     "(no captured graph replay)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
