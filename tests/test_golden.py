@@ -137,6 +137,47 @@ def test_gpu_plain_headline_commit_matches_reference_fixture(zk):
     check_commit(rec, d_leaves.to_host(), d_dig.to_host(nd * 4), d_cap.to_host())
 
 
+@pytest.mark.gpu
+def test_gpu_eight_resident_400x2e20_jobs(zk):
+    """Eight 400 x 2^20 rate-1 Poseidon2 jobs resident at once (about 82 GB of
+    coefficients, leaf rows and digests) in ONE zkx_commit_batch.  Job 0 holds
+    the fixture's coefficients: its leaf rows, digests and caps are the
+    fixture's.  Jobs 1-7 hold device-generated coefficients: each one's
+    digests and caps equal those of the plain C-ABI sequence
+    (compute_batched_lde -> compute_transpose_rev -> fill_digests on the
+    device) over the same coefficients; their leaf rows are covered by those
+    digests, each leaf digest being the hash of its row."""
+    rec = next(r for r in FIX["commits"] if r["cols"] == 400)
+    zk.init_coset_rs(0, 24, 7)
+    cols, lg_n, rate, h, ht = rec["cols"], rec["lg_n"], rec["rate"], rec["cap_height"], rec["hash"]
+    lg, N = lg_n + rate, 1 << (lg_n + rate)
+    nc, nd = 1 << h, 2 * (N - (1 << h))
+    commits = [zk.Commit(ob.canonical(cols << lg_n, rec["seed"]), cols, lg_n, rate, h, ht)]
+    for j in range(1, 8):
+        x = zk.DeviceBuffer(cols << lg_n)
+        x.fill_random(4000 + j)
+        commits.append(zk.Commit(x, cols, lg_n, rate, h, ht))
+    print(f"one zkx_commit_batch of 8 resident {commit_id(rec)} jobs", flush=True)
+    zk.commit_batch(commits)
+    got = [(sha(c.d_digests.to_host(nd * 4)), c.d_caps.to_host()) for c in commits]
+    assert sha(commits[0].d_leaves.to_host()) == rec["leaves_sha256"], "job 0 leaf rows"
+    assert got[0][0] == rec["digests_sha256"], "job 0 digests"
+    assert np.array_equal(got[0][1], u64s(rec["caps"])), "job 0 caps"
+    for c in commits[1:]:  # the plain sequence below needs their leaf-row memory
+        c.d_leaves.free()
+    d_lde, d_leaves = zk.DeviceBuffer(cols * N), zk.DeviceBuffer(cols * N)
+    d_dig, d_cap = zk.DeviceBuffer(nd * 4), zk.DeviceBuffer(nc * 4)
+    for j, c in enumerate(commits[1:], start=1):
+        print(f"job {j}: plain C-ABI sequence over the same coefficients", flush=True)
+        zk.lde_batch(0, d_lde, c.d_coeffs, lg_n, zk.NTTConfig(batches=cols, extension_rate_bits=rate, with_coset=True,
+                                                             are_inputs_on_device=True, are_outputs_on_device=True))
+        zk.transpose_rev_batch(0, d_leaves, d_lde, lg, zk.TransposeConfig(batches=cols, are_inputs_on_device=True,
+                                                                          are_outputs_on_device=True))
+        zk.fill_digests_buf_linear_gpu_with_gpu_ptr(d_dig, d_cap, d_leaves, nd, nc, N, cols, h, ht, 0)
+        assert sha(d_dig.to_host(nd * 4)) == got[j][0], f"job {j} digests"
+        assert np.array_equal(d_cap.to_host(), got[j][1]), f"job {j} caps"
+
+
 # ------------------------------------- benchmark workloads (workloads.json)
 
 WL = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "workloads.json")))
