@@ -41,25 +41,28 @@ HBM_PEAK_GBS = 8000.0
 # Per-unit figures of the leaf-hash launch on this workload, measured with
 # rocprofv3 --pmc on the batched 135 x 2^15 rate-3 launch (4 commits) and
 # kept in profiles/merkle_leaves_valu_135x2e18.txt and
-# profiles/hbm_traffic_135x2e15_rate3.txt:
-#   VALU and MFMA wave-instructions per Poseidon permutation (SQ_INSTS_VALU,
-#   SQ_INSTS_MFMA over permutations; leaf loads and in-tile levels included),
-#   by Merkle work-group size (log2 lanes).  The leaf permutations run their
-#   MDS layers on the matrix cores (v_mfma_i32_32x32x32_i8).
+# profiles/hbm_traffic_135x2e15_rate3.txt.  A batch of that size builds its
+# tree levels in launches of their own (zkx_set_level_launches), so the leaf
+# launch hashes the leaves only:
+#   VALU and MFMA wave-instructions per leaf permutation (SQ_INSTS_VALU,
+#   SQ_INSTS_MFMA over the launch's 17 permutations per leaf; leaf loads and
+#   leaf-row writes included), by Merkle work-group size (log2 lanes).  The
+#   permutations run their MDS layers on the matrix cores
+#   (v_mfma_i32_32x32x32_i8).
 #   keyed by (merkle_tile_log, merkle_valu)
-VALU_WAVE_INSTS_PER_PERM = {(8, 0): 236.78, (9, 0): 231.96, (10, 0): 229.52, (9, 1): 310.05}
-MFMA_WAVE_INSTS_PER_PERM = {(8, 0): 7.969, (9, 0): 7.969, (10, 0): 7.969, (9, 1): 0.0}
+VALU_WAVE_INSTS_PER_PERM = {(8, 0): 220.24, (9, 0): 220.19, (10, 0): 220.19, (9, 1): 302.88}
+MFMA_WAVE_INSTS_PER_PERM = {(8, 0): 8.437, (9, 0): 8.437, (10, 0): 8.437, (9, 1): 0.0}
 #   SHA-256 (first 16 hex digits) of the leaf kernel's sources those counts
 #   were measured on (leaf_kernel_digest()); a build whose sources differ is
 #   flagged in the JSON ("counters_match_build": false)
-LEAF_COUNTERS_SOURCES = "02983ad8b7021e36"
+LEAF_COUNTERS_SOURCES = "436678e9dd45d2db"
 #   issue slots (one per 4 cycles) that one such MFMA takes from the vector
 #   issue of its SIMD beside full-rate VALU work: measured live by the
 #   "issue" leg (zkx_issue_probe); this value only when that leg is skipped
 MFMA_ISSUE_SLOTS = 3.25
 #   HBM bytes per commit: FETCH_SIZE x 2 (the counter reads half the bytes of
 #   a plain 1 GiB copy) + WRITE_SIZE x 1, KiB x 1024, by work-group size
-LEAF_TRAFFIC_BYTES_PER_COMMIT = {(8, 0): 8.175e8, (9, 0): 8.089e8, (10, 0): 8.117e8, (9, 1): 8.509e8}
+LEAF_TRAFFIC_BYTES_PER_COMMIT = {(8, 0): 7.723e8, (9, 0): 7.786e8, (10, 0): 8.055e8, (9, 1): 7.827e8}
 # Sources of the leaf-hash launch (zeknox_amd/csrc)
 LEAF_KERNEL_SOURCES = ("gl64.h", "gl_constants.h", "gmem.h", "hash.h", "poseidon_mfma.h", "merkle_kernels.h",
                        "merkle_launch.h", "merkle_launch_impl.h", "merkle_types.h", "merkle_poseidon.hip")
@@ -99,6 +102,9 @@ def parse():
     ap.add_argument("--jobs", type=int, default=32, help="135 x 2^15 rate-3 commits per batched step")
     ap.add_argument("--no-graphs", action="store_true", help="batched steps launched kernel by kernel "
                     "(no captured graph replay)")
+    ap.add_argument("--merkle-schedule", choices=("auto", "fused", "by_level"), default="auto",
+                    help="Merkle schedule of batches: the library's rule (auto: tree levels launched apart from "
+                         "2^18 leaves), always fused into the leaf launch, or always one launch per level")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="served-stream duration (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0,
@@ -638,6 +644,7 @@ def main():
     # call: replay is used only when the warm-up steps cover that capture)
     graphs = not args.no_graphs and args.warmup >= 3
     zk.set_commit_graphs(graphs)
+    zk.set_level_launches({"auto": zk.LEVEL_LAUNCHES_DEFAULT, "fused": None, "by_level": 0}[args.merkle_schedule])
     commits = [zk.Commit(d_in, COLS, LG_N, RATE, CAP_H, HASH) for _ in range(args.jobs)]
     if args.schedule:
         zk.set_tuning(zk.Tuning(*[int(v) for v in args.schedule.split(",")]), commits[0])
@@ -681,16 +688,20 @@ def main():
     # stream); peak = VALU_PEAK_TOPS (one wave-instruction per 4 cycles).
     dom = max(batch_kernels, key=lambda k: batch_kernels[k]["ms_per_step"])
     leaf = batch_kernels.get("merkle_leaves")
+    # the batch builds its tree levels in launches of their own (the
+    # library's default for batches of >= LEVEL_LAUNCHES_DEFAULT leaves)
+    levels_apart = args.merkle_schedule == "by_level" or (args.merkle_schedule == "auto"
+                                                          and args.jobs * N >= zk.LEVEL_LAUNCHES_DEFAULT)
     roofline = None
-    if leaf and (schedule["merkle_tile_log"], schedule["merkle_valu"]) in VALU_WAVE_INSTS_PER_PERM:
+    if leaf and levels_apart and (schedule["merkle_tile_log"], schedule["merkle_valu"]) in VALU_WAVE_INSTS_PER_PERM:
         tl = (schedule["merkle_tile_log"], schedule["merkle_valu"])
-        perms = args.jobs * perms_per_commit(N, COLS, nc)
+        perms = args.jobs * N * ((COLS + 7) // 8)  # the leaf sponges
         t_launch = leaf["avg_launch_ms"] * 1e-3
         mslots = issue["mfma_issue_slots"] if issue else MFMA_ISSUE_SLOTS
         valu = perms * VALU_WAVE_INSTS_PER_PERM[tl] * 64 / t_launch / 1e12
         mfma = perms * mslots * MFMA_WAVE_INSTS_PER_PERM[tl] * 64 / t_launch / 1e12
         achieved = valu + mfma
-        alg_bytes = args.jobs * (8 * COLS * N * 2 + 32 * nd)  # LDE read, leaf rows + digests written
+        alg_bytes = args.jobs * (8 * COLS * N * 2 + 32 * N)  # LDE read, leaf rows + leaf digests written
         traffic = LEAF_TRAFFIC_BYTES_PER_COMMIT.get(tl)
         digest = leaf_kernel_digest()
         roofline = {"bound": "valu+mfma issue", "achieved": round(achieved, 2), "peak": round(VALU_PEAK_TOPS, 2),
@@ -707,6 +718,17 @@ def main():
                     "mfma_wave_insts_per_perm": MFMA_WAVE_INSTS_PER_PERM[tl], "mfma_issue_slots": mslots,
                     "mfma_issue_slots_source": "measured (zkx_issue_probe)" if issue else "constant",
                     "leaf_kernel_sources": digest, "counters_match_build": digest == LEAF_COUNTERS_SOURCES}
+
+    # the tree levels: one launch per level over every tree (internal nodes
+    # and caps: N - caps permutations per commit)
+    lv = batch_kernels.get("merkle_levels")
+    merkle_levels = None
+    if lv:
+        lperms = args.jobs * (N - nc)
+        merkle_levels = {"launches_per_step": lv["launches_per_step"], "ms_per_step": round(lv["ms_per_step"], 4),
+                         "perms_per_step": lperms, "perms_per_s": round(lperms / (lv["ms_per_step"] * 1e-3)),
+                         "note": "one launch per level over all trees; the narrowest levels are bound by one "
+                                 "permutation's latency"}
 
     # LDE passes against HBM: algorithmic bytes of the LDE (read n, write N
     # per column) over the batch's NTT launches
@@ -791,7 +813,7 @@ def main():
         "config": {"workload": "LDE+commit: 135 polys x 2^15, rate_bits 3 (2^18-point coset LDE, shift 7), "
                                "transpose to 2^18 leaves x 135, Poseidon Merkle, cap_height 4",
                    "path": mode, "commits_per_step": args.jobs, "parallelism": "single GPU",
-                   "graphs": graphs,
+                   "graphs": graphs, "merkle_levels_apart": levels_apart,
                    "schedule": schedule},
         "plain_commits_per_s": round(1.0 / plain_s, 2),
         "batched_over_plain": round(plain_s / batched_s, 3),
@@ -800,6 +822,7 @@ def main():
         "kernels_batched": batch_kernels,
         "dominant_kernel": dom,
         "roofline": roofline,
+        "merkle_levels": merkle_levels,
         "issue_rates": issue,
         "ntt_roofline": ntt_roofline,
         "cpu_baseline": cpu,

@@ -166,6 +166,13 @@ RustError zkx_set_autotune(size_t device_id, int on);
  * re-staged each call); off: every batch is launched kernel by kernel.  No
  * setting changes a result. */
 RustError zkx_set_commit_graphs(size_t device_id, int on);
+/* Merkle schedule of zkx_commit_batch and the server's engines: a batch
+ * whose trees hold at least min_leaves leaves in total (default 2^18) hashes
+ * the leaves in one launch per hash kind and builds the levels above them
+ * with one launch per level over every tree; smaller batches finish every
+ * tree inside the leaf launch.  0: always by level; UINT64_MAX: never.  No
+ * setting changes a result. */
+RustError zkx_set_level_launches(size_t device_id, uint64_t min_leaves);
 /* Graphs the device's zkx_commit_batch engine holds (tests, benchmarks). */
 size_t zkx_commit_graphs_captured(size_t device_id);
 /* Stage counts of the DIF passes over 2^lg points under `tuning` (NULL: the

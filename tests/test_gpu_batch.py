@@ -50,17 +50,33 @@ MIX = [(2, 10, 1, 4, 0), (20, 11, 2, 4, 3), (80, 10, 3, 4, 0), (135, 9, 1, 4, 3)
 PIECES_MIX = MIX + [(8, 15, 2, 4, 0), (6, 15, 3, 4, 3), (4, 16, 3, 4, 0)]
 
 
-@pytest.mark.parametrize("ntt_tiles,merkle_groups", [(1, 1), (5, 3), (64, 7), (0, 2), (3, 0)])
-def test_batch_in_pieces_vs_oracle(zk, ntt_tiles, merkle_groups):
+# Merkle schedules of a batch (zkx_set_level_launches): the trees finished
+# inside the leaf launch, or every level above the leaves one launch over all
+# trees; the default picks by the batch's total leaves
+LEVELS = {"fused": None, "by_level": 0}
+
+
+@pytest.fixture
+def merkle_schedule(zk, request):
+    zk.set_level_launches(LEVELS[request.param])
+    yield request.param
+    zk.set_level_launches(zk.LEVEL_LAUNCHES_DEFAULT)
+
+
+@pytest.mark.parametrize("merkle_schedule,ntt_tiles,merkle_groups",
+                         [("fused", 1, 1), ("fused", 5, 3), ("fused", 64, 7), ("fused", 0, 2), ("fused", 3, 0),
+                          ("by_level", 1, 1), ("by_level", 0, 3)], indirect=["merkle_schedule"])
+def test_batch_in_pieces_vs_oracle(zk, merkle_schedule, ntt_tiles, merkle_groups):
     """zkx_commit_batch_pieces: every NTT pass launch split into ranges of
     ntt_tiles tiles and every Merkle launch into ranges of merkle_groups
-    work-groups gives the oracle's results (and so the unsplit batch's)."""
+    work-groups gives the oracle's results (and so the unsplit batch's),
+    under both Merkle schedules."""
     commits, inputs = [], []
     for i, (cols, lg_n, rate, h, ht) in enumerate(PIECES_MIX):
         x = ob.canonical(cols << lg_n, 900 + i)
         inputs.append(x)
         commits.append(zk.Commit(x, cols, lg_n, rate, h, ht))
-    print(f"pieces ntt_tiles={ntt_tiles} merkle_groups={merkle_groups}", flush=True)
+    print(f"pieces {merkle_schedule} ntt_tiles={ntt_tiles} merkle_groups={merkle_groups}", flush=True)
     zk.commit_batch(commits, ntt_tiles=ntt_tiles, merkle_groups=merkle_groups)
     for (cols, lg_n, rate, h, ht), x, c in zip(PIECES_MIX, inputs, commits):
         leaves, dig, cap = c.results()
@@ -70,7 +86,10 @@ def test_batch_in_pieces_vs_oracle(zk, ntt_tiles, merkle_groups):
         assert np.array_equal(cap, ec)
 
 
-def test_batch_mixed_vs_oracle(zk):
+@pytest.mark.parametrize("merkle_schedule", ["fused", "by_level"], indirect=True)
+def test_batch_mixed_vs_oracle(zk, merkle_schedule):
+    """The mix (all five hashes, caps == leaves, one-leaf subtrees, leaves
+    of at most 4 elements) under both Merkle schedules."""
     commits, inputs = [], []
     for i, (cols, lg_n, rate, h, ht) in enumerate(MIX):
         x = ob.canonical(cols << lg_n, 500 + i)
@@ -78,7 +97,7 @@ def test_batch_mixed_vs_oracle(zk):
         commits.append(zk.Commit(x, cols, lg_n, rate, h, ht))
     zk.commit_batch(commits)
     for (cols, lg_n, rate, h, ht), x, c in zip(MIX, inputs, commits):
-        print(f"job cols={cols} lg_n={lg_n} rate={rate} h={h} ht={ht}", flush=True)
+        print(f"{merkle_schedule} job cols={cols} lg_n={lg_n} rate={rate} h={h} ht={ht}", flush=True)
         leaves, dig, cap = c.results()
         el, ed, ec = oracle_commit(x, cols, lg_n, rate, h, ht)
         assert np.array_equal(leaves, el)

@@ -157,6 +157,7 @@ def _load():
         "zkx_set_tuning": (RustError, [sz, vp, ctypes.POINTER(Tuning)]),
         "zkx_set_autotune": (RustError, [sz, ctypes.c_int]),
         "zkx_set_commit_graphs": (RustError, [sz, ctypes.c_int]),
+        "zkx_set_level_launches": (RustError, [sz, u64]),
         "zkx_commit_graphs_captured": (sz, [sz]),
         "zkx_tuning_clear": (RustError, [sz]),
         "zkx_tuning_save": (RustError, [sz, ctypes.c_char_p]),
@@ -464,6 +465,16 @@ def set_autotune(on, device_id=0):
 def set_commit_graphs(on, device_id=0):
     """Replay repeated batch launch lists as captured HIP graphs (default on)."""
     _check(lib.zkx_set_commit_graphs(device_id, 1 if on else 0))
+
+
+LEVEL_LAUNCHES_DEFAULT = 1 << 18
+
+
+def set_level_launches(min_leaves, device_id=0):
+    """Batches whose trees hold at least min_leaves leaves build the tree
+    levels with one launch per level (0: always, None: never)."""
+    v = (1 << 64) - 1 if min_leaves is None else int(min_leaves)
+    _check(lib.zkx_set_level_launches(device_id, v))
 
 
 def commit_graphs_captured(device_id=0):

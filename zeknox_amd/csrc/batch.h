@@ -9,10 +9,13 @@
 //
 // An Engine runs any number of uneven jobs together on one stream: one
 // launch per NTT pass index covers that pass of every job, one launch per
-// hash kind and work-group size builds every tree (the upper levels of each
-// tree are finished inside the same launch by the last work-group of each
-// group of tiles), so small jobs share the GPU and the latency-bound top
-// levels of different trees overlap with other trees' leaf hashing.  The LDE stays in HBM in its natural DIF output order
+// hash kind and work-group size hashes every tree's leaves.  In a small
+// batch the same launch finishes the trees (the upper levels of each tree
+// by the last work-group of each group of tiles), so the latency-bound top
+// levels of different trees overlap with other trees' leaf hashing; a batch
+// whose trees hold at least Device::level_launch_leaves leaves in total
+// builds the levels with one launch per level over every tree instead, each
+// level at full occupancy.  The LDE stays in HBM in its natural DIF output order
 // (bit-reversed, column-major) and the leaf kernels read it there directly:
 // no bit-reversal or transpose pass; the leaf rows are written by the leaf
 // kernel as it reads them.
@@ -91,7 +94,7 @@ private:
         size_t offset;      // bytes into desc_
         uint32_t count;
         uint64_t blocks;
-        int kind;           // 0 NTT, 1 Merkle
+        int kind;           // 0 NTT, 1 Merkle (leaves, or fused trees), 2 Merkle level
         int hash;           // NTT: contiguous; Merkle: hash type
         int kernel;         // NTT: ntt_kernel_kind of the list
         uint32_t tile_log;  // Merkle work-group size (log2 lanes)

@@ -140,6 +140,15 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
         if (built) dev_.sync();
     }
 
+    // ---- Merkle schedule: batches whose trees hold many leaves build the
+    // levels above the leaves with one launch per level over every tree
+    // (after a leaf launch that writes the leaf digests only); smaller ones
+    // finish every tree inside its hash kind's leaf launch
+    uint64_t tree_leaves = 0;
+    for (size_t i = 0; i < n; i++)
+        if (jobs[i].cap_height < jobs[i].lg_n + jobs[i].rate) tree_leaves += 1ull << (jobs[i].lg_n + jobs[i].rate);
+    const bool by_level = tree_leaves > 0 && tree_leaves >= dev_.level_launch_leaves.load();
+
     // ---- plans
     std::vector<std::vector<NttPass>> ntt_plan(n);
     std::vector<uint64_t> ntt_tiles(n);
@@ -170,7 +179,12 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
         std::vector<MerkleArgs> passes;
         plan_merkle(s, passes, tun[i].merkle_tile_log);
         mk_ctr[i] = total_ctr;
-        total_ctr += fuse_merkle_passes(passes, mk[i]);
+        if (by_level) {
+            mk[i] = passes[0];  // the leaf pass alone: its levels are launched apart
+            mk[i].leaves_only = 1;
+        } else {
+            total_ctr += fuse_merkle_passes(passes, mk[i]);
+        }
         max_ntt = std::max(max_ntt, ntt_plan[i].size());
     }
 
@@ -227,7 +241,7 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
                 if ((int)jobs[i].hash_type != ht || tun[i].merkle_tile_log != tl) continue;
                 MerkleArgs a = mk[i];
                 a.tile_begin = blocks;
-                a.ctr = ctr_.as<uint32_t>() + mk_ctr[i];
+                a.ctr = by_level ? nullptr : ctr_.as<uint32_t>() + mk_ctr[i];
                 blocks += a.count >> a.tile_log;
                 list.push_back(a);
             }
@@ -236,6 +250,34 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
             launches.push_back({off, (uint32_t)list.size(), blocks, 1, ht, 0, tl});
             host.resize(align_up(host.size()));
         }
+    if (by_level) {
+        // level k above the leaves of every tree that has it, one launch per
+        // hash kind; mk[i].lg_w is the log2 width of tree i's leaf level per
+        // subtree (0: caps == leaves, no levels)
+        uint32_t depth = 0;
+        for (size_t i = 0; i < n; i++) depth = std::max(depth, mk[i].lg_w);
+        for (uint32_t k = 1; k <= depth; k++)
+            for (int ht : {kPoseidon, kKeccak, kPoseidonBN128, kPoseidon2, kMonolith}) {
+                std::vector<MerkleArgs> list;
+                uint64_t blocks = 0;
+                for (size_t i = 0; i < n; i++) {
+                    if ((int)jobs[i].hash_type != ht || mk[i].lg_w < k) continue;
+                    MerkleArgs a{};
+                    a.dig = mk[i].dig;
+                    a.cap = mk[i].cap;
+                    a.sdl = mk[i].sdl;
+                    a.lg_w = mk[i].lg_w - k;  // the level written (0: the caps)
+                    a.count = (mk[i].count >> mk[i].lg_w) << a.lg_w;
+                    a.tile_begin = blocks;
+                    blocks += (a.count + kMerkleLevelLanes - 1) / kMerkleLevelLanes;
+                    list.push_back(a);
+                }
+                if (list.empty()) continue;
+                const size_t off = push(list.data(), list.size() * sizeof(MerkleArgs));
+                launches.push_back({off, (uint32_t)list.size(), blocks, 2, ht, 0, 0});
+                host.resize(align_up(host.size()));
+            }
+    }
     size_t off0 = 0;
     const void *desc_before = desc_.p;
     stage(host.data(), host.size(), off0);
@@ -307,9 +349,12 @@ void Engine::issue(hipStream_t s, const std::vector<Launch> &launches, const Lau
         if (l.kind == 0) {
             launch_ntt_list(s, reinterpret_cast<const NttPass *>(base), l.count, l.blocks, l.hash == 1, l.kernel,
                             "batch_ntt", pieces.ntt_tiles);
-        } else {
+        } else if (l.kind == 1) {
             launch_merkle_pass(s, l.hash, reinterpret_cast<const MerkleArgs *>(base), l.count, MerkleArgs{},
                                l.blocks, l.tile_log, pieces.merkle_groups);
+        } else {
+            launch_merkle_level(s, l.hash, reinterpret_cast<const MerkleArgs *>(base), l.count, l.blocks,
+                                pieces.merkle_groups);
         }
     }
 }

@@ -470,6 +470,11 @@ RustError zkx_set_commit_graphs(size_t device_id, int on)
     return guarded([&] { device(device_id).commit_graphs = on != 0; });
 }
 
+RustError zkx_set_level_launches(size_t device_id, uint64_t min_leaves)
+{
+    return guarded([&] { device(device_id).level_launch_leaves = min_leaves; });
+}
+
 size_t zkx_commit_graphs_captured(size_t device_id)
 {
     try {

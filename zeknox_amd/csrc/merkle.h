@@ -40,6 +40,11 @@ uint64_t fuse_merkle_passes(const MerkleArgs *passes, uint32_t n, MerkleArgs &ou
 void launch_merkle_pass(hipStream_t stream, int hash_type, const MerkleArgs *dev_list,
                         uint32_t n, const MerkleArgs &one, uint64_t blocks, uint32_t tile_log_max,
                         uint64_t chunk = 0);
+// One launch of the level kernel (merkle_kernels.h k_merkle_level) over n
+// level descriptors in device memory; `blocks` work-groups of 256 lanes in
+// total, issued in ranges of at most `chunk` work-groups when chunk > 0.
+void launch_merkle_level(hipStream_t stream, int hash_type, const MerkleArgs *dev_list, uint32_t n, uint64_t blocks,
+                         uint64_t chunk = 0);
 // The whole tree in one launch on the device stream; call with d.mu held.
 void merkle_build(Device &d, const MerkleSpec &s);
 

@@ -122,6 +122,26 @@ void launch_merkle_pass(hipStream_t stream, int hash_type, const MerkleArgs *dev
     }
 }
 
+void launch_merkle_level(hipStream_t stream, int hash_type, const MerkleArgs *dev_list, uint32_t n, uint64_t blocks,
+                         uint64_t chunk)
+{
+    if (blocks == 0) return;
+    if (blocks > 0x7FFFFFFFull) invalid("tree level too large for one launch");
+    ProfScope ps("merkle_levels", stream);
+    const uint64_t step = chunk ? chunk : blocks;
+    for (uint64_t b0 = 0; b0 < blocks; b0 += step) {
+        const uint64_t nb = std::min(step, blocks - b0);
+        switch (hash_type) {
+        case kPoseidon2: launch_merkle_level_hash<kPoseidon2>(stream, dev_list, n, nb, (uint32_t)b0); break;
+        case kKeccak: launch_merkle_level_hash<kKeccak>(stream, dev_list, n, nb, (uint32_t)b0); break;
+        case kMonolith: launch_merkle_level_hash<kMonolith>(stream, dev_list, n, nb, (uint32_t)b0); break;
+        case kPoseidonBN128: launch_merkle_level_hash<kPoseidonBN128>(stream, dev_list, n, nb, (uint32_t)b0); break;
+        default: launch_merkle_level_hash<kPoseidon>(stream, dev_list, n, nb, (uint32_t)b0); break;
+        }
+        ZKX_HIP(hipGetLastError());
+    }
+}
+
 void merkle_build(Device &d, const MerkleSpec &s)
 {
     check_merkle(s);

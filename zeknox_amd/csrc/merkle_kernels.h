@@ -171,7 +171,7 @@ k_merkle_tile(const MerkleArgs *__restrict__ jobs, uint32_t njobs, MerkleArgs on
 #pragma unroll
         for (int i = 0; i < 4; i++) dst[i] = h[i];
     }
-    if (lg_w == 0) return;  // caps == leaves: done
+    if (lg_w == 0 || a.leaves_only) return;  // caps == leaves, or levels launched apart
     __syncthreads();
 
     while (true) {
@@ -256,6 +256,71 @@ k_merkle_tile(const MerkleArgs *__restrict__ jobs, uint32_t njobs, MerkleArgs on
         }
         cur = half;
         __syncthreads();
+    }
+}
+
+// ---- one level of many trees per launch
+//
+// For large batches the trees' levels above the leaves run as one launch per
+// level over every tree (after a leaves_only leaf launch): a level of n
+// nodes is then n / 256 work-groups of full waves, where the fused kernel
+// reduces each tile's levels on ever fewer lanes, one permutation latency
+// each, holding the work-group's slot meanwhile.  The descriptor's lg_w is
+// the OUTPUT level here (width 2^lg_w per subtree; 0: the caps), count its
+// nodes over all subtrees, tile_begin its first work-group; node g =
+// hash_two of the 8 consecutive words of its two children at level lg_w + 1.
+// Poseidon hashes 64 nodes per wave on the matrix cores (the state [l, r, 0]
+// laid out as leaf_hash_mfma lays out a leaf's first chunk), the other hashes
+// one node per lane.
+template <int HT>
+__global__ void __launch_bounds__(kMerkleLevelLanes) __attribute__((amdgpu_waves_per_eu(merkle_waves_per_eu(HT))))
+k_merkle_level(const MerkleArgs *__restrict__ jobs, uint32_t njobs, uint32_t block0)
+{
+    __shared__ uint64_t kt[HT == kPoseidon ? kMfmaRcWords : 1];
+    const uint32_t bid = blockIdx.x + block0;
+    const MerkleArgs a = *find_job(jobs, njobs, bid);
+    const uint32_t t = threadIdx.x, lw = a.lg_w;
+    const uint64_t g0 = (uint64_t)(bid - a.tile_begin) * kMerkleLevelLanes;
+    const uint64_t wmask = (1ull << lw) - 1;
+    // first word of node g's two children (level lw + 1 >= 1)
+    auto children = [&](uint64_t g) { return node_ptr(a, g >> lw, lw + 1, 2 * (g & wmask)); };
+    const uint64_t g = g0 + t;
+    uint64_t out[4];
+    if constexpr (HT == kPoseidon) {
+        mfma_fill_rc(kt, t, kMerkleLevelLanes);
+        __syncthreads();
+        const uint32_t w = t >> 6, j = t & 31, h = (t >> 5) & 1;
+        const MfmaOps o = mfma_setup();
+        uint64_t s[2][6];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const uint64_t n = g0 + 64 * w + 32 * q + j;
+            const uint32_t el0 = 6 * ((h + q) & 1);  // this lane's first element of group q
+            const bool real = n < a.count;           // lanes past the level's end hash zeros
+            const gptr<const uint64_t> c = children(real ? n : 0);
+#pragma unroll
+            for (int e = 0; e < 6; e++) s[q][e] = real && el0 + e < 8 ? c[el0 + e] : 0;
+        }
+        poseidon_permute_mfma(s, o, kt);
+        // elements 0-3 of group h sit in slots 0-3 of this lane: node g0 + 64w + 32h + j = g
+#pragma unroll
+        for (int i = 0; i < 4; i++) out[i] = gl::canon(h ? s[1][i] : s[0][i]);
+    } else {
+        if (g < a.count) {
+            const gptr<const uint64_t> c = children(g);
+            uint64_t l[4], r[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                l[i] = c[i];
+                r[i] = c[4 + i];
+            }
+            hash_two<HT>(l, r, out);
+        }
+    }
+    if (g < a.count) {
+        const gptr<uint64_t> d = lw ? node_ptr(a, g >> lw, lw, g & wmask) : gmem(a.cap) + 4 * g;
+#pragma unroll
+        for (int i = 0; i < 4; i++) d[i] = out[i];
     }
 }
 

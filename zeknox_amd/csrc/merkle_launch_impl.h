@@ -18,4 +18,12 @@ void launch_merkle_hash(hipStream_t stream, const MerkleArgs *dev_list, uint32_t
     else hipLaunchKernelGGL((k_merkle_tile<HT, 256>), g, dim3(256), 0, stream, dev_list, n, one, block0);
 }
 
+template <int HT>
+void launch_merkle_level_hash(hipStream_t stream, const MerkleArgs *dev_list, uint32_t n, uint64_t blocks,
+                              uint32_t block0)
+{
+    hipLaunchKernelGGL((k_merkle_level<HT>), dim3((uint32_t)blocks), dim3(kMerkleLevelLanes), 0, stream, dev_list, n,
+                       block0);
+}
+
 }  // namespace zkx

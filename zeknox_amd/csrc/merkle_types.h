@@ -14,6 +14,8 @@ constexpr int kMerkleMinTileLog = 8;
 constexpr int kMerkleMaxTileLog = 10;
 // Passes of one tree: tiles of >= 2^8 nodes reduce 2^32 leaves in 4.
 constexpr int kMerkleMaxPasses = 4;
+// Work-group size (nodes) of the one-level-per-launch kernel.
+constexpr int kMerkleLevelLanes = 256;
 
 struct MerkleArgs {
     const uint64_t *leaves;  // first pass: leaf source
@@ -35,7 +37,7 @@ struct MerkleArgs {
     uint32_t next_ctr[kMerkleMaxPasses - 1];  // first arrival counter of each upper pass
     uint32_t *ctr;           // arrival counters, zero before and after the launch (null: n_next == 0)
     uint32_t leaf_valu;      // Poseidon leaves: 1 = all-VALU permutations (no matrix-core MDS)
-    uint32_t pad_;
+    uint32_t leaves_only;    // 1: the launch writes the leaf digests and stops (levels launched apart)
 };
 
 }  // namespace zkx

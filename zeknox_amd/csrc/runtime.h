@@ -109,6 +109,14 @@ struct Device {
     // Batch engines replay launch lists they have issued before as HIP
     // graphs (zkx_set_commit_graphs); read at each commit.
     std::atomic<bool> commit_graphs{true};
+    // Batches whose trees hold at least this many leaves in total build the
+    // levels above the leaves with one launch per level over all trees
+    // (zkx_set_level_launches); smaller ones, one fused launch per hash kind.
+    // Measured on 1-16 commits of 135 x 2^15 rate 3 and 20 x 2^12 rate 2
+    // (Poseidon, Poseidon2): by level 0.93-0.99 of the fused time from 2^18
+    // leaves up, 1.14-1.16 below (each level launch costs at least one
+    // permutation's latency, about 43 us).
+    std::atomic<uint64_t> level_launch_leaves{1ull << 18};
     Tuning tuning_default();
     // The schedule of the C-ABI Merkle entry points for one tree: the
     // default, with lone_tree_tile_log()'s work-groups unless the caller set
