@@ -112,8 +112,11 @@ __device__ __forceinline__ uint64_t reduce96(uint64_t lo, uint32_t top) { return
 // through inline asm.  gfx950 needs two wait states between a VALU write of
 // a lane mask and a VALU read of it (the compiler inserts the same for its
 // own carry chains): every mask is created and consumed inside one asm
-// block, behind an s_nop 1.  The inline constant -1 is 0xFFFFFFFF = EPS as
-// the 32-bit multiplicand.
+// block, behind an s_nop 1 -- except the product carry of mul128c, which is
+// read by the second asm block of reduce128c_lazy(2): that block depends on
+// the first one, whose inputs depend on the multiply-add writing the mask,
+// so at least the first block's three instructions lie between them.  The
+// inline constant -1 is 0xFFFFFFFF = EPS as the 32-bit multiplicand.
 
 // Lazy residue (below 2^64, congruent mod p) of top * 2^64 + lo, top < 2^32:
 // u = lo + top * EPS (mod 2^64, carry c: one v_mad_u64_u32), then
@@ -145,6 +148,25 @@ __device__ __forceinline__ void mul128(uint64_t a, uint64_t b, uint64_t &lo, uin
     lo = pack(lo32(t0), lo32(t2));
 }
 
+// 64x64 -> 128 product from four multiply-adds, the carry of the middle sum
+// left in a lane mask: a * b = lo + (hi + c * 2^32) * 2^64, c in {0, 1}
+// (any a, b < 2^64).  u = a0 b0; v = a0 b1 + hi(u) (< 2^64); w = a1 b0 + v,
+// whose bit 64 is c; hi = a1 b1 + hi(w) <= 2^64 - 2^32 (no carry).  Two
+// instructions fewer than mul128 (one multiply-add and one register-pair
+// copy): the carry goes into the reduction as a borrow (reduce128c_lazy).
+__device__ __forceinline__ void mul128c(uint64_t a, uint64_t b, uint64_t &lo, uint64_t &hi, uint64_t &c)
+{
+    const uint32_t a0 = lo32(a), a1 = hi32(a), b0 = lo32(b), b1 = hi32(b);
+    const uint64_t u = (uint64_t)a0 * b0;
+    const uint64_t v = (uint64_t)a0 * b1 + hi32(u);
+    uint64_t w;
+    asm("v_mad_u64_u32 %[w], %[c], %[x], %[y], %[v]"
+        : [w] "=&v"(w), [c] "=&s"(c)
+        : [x] "v"(a1), [y] "v"(b0), [v] "v"(v));
+    hi = (uint64_t)a1 * b1 + hi32(w);
+    lo = pack(lo32(u), lo32(w));
+}
+
 // A residue of hi * 2^64 + lo (any 128-bit value) below 2^64 but not
 // necessarily below p ("lazy"):  hi * 2^64 = h0 * EPS - h1 (mod p).
 //   u = lo + h0 * EPS (mod 2^64; carry c)      one v_mad_u64_u32
@@ -173,6 +195,39 @@ __device__ __forceinline__ uint64_t reduce128_lazy(uint64_t lo, uint64_t hi)
         "v_subb_co_u32_e64 %[cb], %[bj], %[cv], 0, %[b]"
         : [zlo] "=&v"(zlo), [zhi] "=&v"(zhi), [cb] "=&v"(cb), [b1] "=&s"(b1), [b] "=&s"(b), [bj] "=&s"(bj)
         : [ulo] "v"(lo32(u)), [uhi] "v"(hi32(u)), [h1] "v"(h1), [cv] "v"(cv));
+    uint64_t t, tj;
+    asm("v_mad_i64_i32 %[t], %[tj], %[cb], -1, %[z]"
+        : [t] "=&v"(t), [tj] "=&s"(tj)
+        : [cb] "v"(cb), [z] "v"(pack(zlo, zhi)));
+    // The high-word add stays in asm: written in C, the compiler folds it
+    // into a following carry chain (canon's hi + 0 + carry becomes
+    // hi(t) + cb + carry), whose carry-out then counts the wrap of this add.
+    uint32_t rhi;
+    asm("v_add_u32 %[r], %[a], %[b]" : [r] "=v"(rhi) : [a] "v"(hi32(t)), [b] "v"(cb));
+    return pack(lo32(t), rhi);
+}
+
+// reduce128_lazy of lo + (hi + cin * 2^32) * 2^64 (cin a lane mask, the
+// high word plus cin below 2^64): cin is the borrow into u - h1.
+__device__ __forceinline__ uint64_t reduce128c_lazy(uint64_t lo, uint64_t hi, uint64_t cin)
+{
+    const uint32_t h0 = lo32(hi), h1 = hi32(hi);
+    uint64_t u, sc;
+    uint32_t cv;
+    asm("v_mad_u64_u32 %[u], %[sc], %[h0], -1, %[lo]\n\t"
+        "s_nop 1\n\t"
+        "v_cndmask_b32_e64 %[cv], 0, 1, %[sc]"
+        : [u] "=&v"(u), [sc] "=&s"(sc), [cv] "=&v"(cv)
+        : [h0] "v"(h0), [lo] "v"(lo));
+    uint32_t zlo, zhi, cb;
+    uint64_t b1, b, bj;
+    asm("v_subb_co_u32_e64 %[zlo], %[b1], %[ulo], %[h1], %[cin]\n\t"
+        "s_nop 1\n\t"
+        "v_subb_co_u32_e64 %[zhi], %[b], %[uhi], 0, %[b1]\n\t"
+        "s_nop 1\n\t"
+        "v_subb_co_u32_e64 %[cb], %[bj], %[cv], 0, %[b]"
+        : [zlo] "=&v"(zlo), [zhi] "=&v"(zhi), [cb] "=&v"(cb), [b1] "=&s"(b1), [b] "=&s"(b), [bj] "=&s"(bj)
+        : [ulo] "v"(lo32(u)), [uhi] "v"(hi32(u)), [h1] "v"(h1), [cv] "v"(cv), [cin] "s"(cin));
     uint64_t t, tj;
     asm("v_mad_i64_i32 %[t], %[tj], %[cb], -1, %[z]"
         : [t] "=&v"(t), [tj] "=&s"(tj)
@@ -228,6 +283,47 @@ __device__ __forceinline__ void reduce128_lazy2(uint64_t lo0, uint64_t hi0, uint
     r1 = pack(lo32(t1), rh1);
 }
 
+// reduce128_lazy2 with the borrows cin0, cin1 (reduce128c_lazy).
+__device__ __forceinline__ void reduce128c_lazy2(uint64_t lo0, uint64_t hi0, uint64_t cin0, uint64_t lo1, uint64_t hi1,
+                                                 uint64_t cin1, uint64_t &r0, uint64_t &r1)
+{
+    uint64_t u0, u1, sc0, sc1;
+    uint32_t cv0, cv1;
+    asm("v_mad_u64_u32 %[u0], %[sc0], %[a0], -1, %[l0]\n\t"
+        "v_mad_u64_u32 %[u1], %[sc1], %[a1], -1, %[l1]\n\t"
+        "s_nop 0\n\t"
+        "v_cndmask_b32_e64 %[cv0], 0, 1, %[sc0]\n\t"
+        "v_cndmask_b32_e64 %[cv1], 0, 1, %[sc1]"
+        : [u0] "=&v"(u0), [u1] "=&v"(u1), [sc0] "=&s"(sc0), [sc1] "=&s"(sc1), [cv0] "=&v"(cv0), [cv1] "=&v"(cv1)
+        : [a0] "v"(lo32(hi0)), [l0] "v"(lo0), [a1] "v"(lo32(hi1)), [l1] "v"(lo1));
+    uint32_t zl0, zh0, cb0, zl1, zh1, cb1;
+    uint64_t b10, b0, j0, b11, b1, j1;
+    asm("v_subb_co_u32_e64 %[zl0], %[b10], %[ul0], %[h10], %[ci0]\n\t"
+        "v_subb_co_u32_e64 %[zl1], %[b11], %[ul1], %[h11], %[ci1]\n\t"
+        "s_nop 0\n\t"
+        "v_subb_co_u32_e64 %[zh0], %[b0], %[uh0], 0, %[b10]\n\t"
+        "v_subb_co_u32_e64 %[zh1], %[b1], %[uh1], 0, %[b11]\n\t"
+        "s_nop 0\n\t"
+        "v_subb_co_u32_e64 %[cb0], %[j0], %[cv0], 0, %[b0]\n\t"
+        "v_subb_co_u32_e64 %[cb1], %[j1], %[cv1], 0, %[b1]"
+        : [zl0] "=&v"(zl0), [zh0] "=&v"(zh0), [cb0] "=&v"(cb0), [zl1] "=&v"(zl1), [zh1] "=&v"(zh1),
+          [cb1] "=&v"(cb1), [b10] "=&s"(b10), [b0] "=&s"(b0), [j0] "=&s"(j0), [b11] "=&s"(b11), [b1] "=&s"(b1),
+          [j1] "=&s"(j1)
+        : [ul0] "v"(lo32(u0)), [uh0] "v"(hi32(u0)), [h10] "v"(hi32(hi0)), [cv0] "v"(cv0), [ul1] "v"(lo32(u1)),
+          [uh1] "v"(hi32(u1)), [h11] "v"(hi32(hi1)), [cv1] "v"(cv1), [ci0] "s"(cin0),
+          [ci1] "s"(cin1));
+    uint64_t t0, t1, tj0, tj1;
+    asm("v_mad_i64_i32 %[t0], %[tj0], %[cb0], -1, %[z0]\n\t"
+        "v_mad_i64_i32 %[t1], %[tj1], %[cb1], -1, %[z1]"
+        : [t0] "=&v"(t0), [t1] "=&v"(t1), [tj0] "=&s"(tj0), [tj1] "=&s"(tj1)
+        : [cb0] "v"(cb0), [z0] "v"(pack(zl0, zh0)), [cb1] "v"(cb1), [z1] "v"(pack(zl1, zh1)));
+    uint32_t rh0, rh1;  // high-word adds kept in asm (see reduce128_lazy)
+    asm("v_add_u32 %[r], %[a], %[b]" : [r] "=v"(rh0) : [a] "v"(hi32(t0)), [b] "v"(cb0));
+    asm("v_add_u32 %[r], %[a], %[b]" : [r] "=v"(rh1) : [a] "v"(hi32(t1)), [b] "v"(cb1));
+    r0 = pack(lo32(t0), rh0);
+    r1 = pack(lo32(t1), rh1);
+}
+
 // x mod p for any x < 2^64 (x >= p iff x + EPS carries).
 __device__ __forceinline__ uint64_t canon(uint64_t x)
 {
@@ -240,6 +336,16 @@ __device__ __forceinline__ uint64_t canon(uint64_t x)
 // a * b, lazy (below 2^64, congruent mod p) for any a, b < 2^64.
 __device__ __forceinline__ uint64_t mul_lazy(uint64_t a, uint64_t b)
 {
+    uint64_t lo, hi, c;
+    mul128c(a, b, lo, hi, c);
+    return reduce128c_lazy(lo, hi, c);
+}
+
+// mul_lazy through mul128 and reduce128_lazy: two instructions more, for
+// kernels where the carry form's schedule spills (its products' operands
+// and partial sums live longer).
+__device__ __forceinline__ uint64_t mul_lazy_nc(uint64_t a, uint64_t b)
+{
     uint64_t lo, hi;
     mul128(a, b, lo, hi);
     return reduce128_lazy(lo, hi);
@@ -250,15 +356,16 @@ __device__ __forceinline__ uint64_t mul_lazy(uint64_t a, uint64_t b)
 __device__ __forceinline__ void mul_lazy2(uint64_t a0, uint64_t b0, uint64_t a1, uint64_t b1, uint64_t &r0,
                                           uint64_t &r1)
 {
-    uint64_t lo0, hi0, lo1, hi1;
-    mul128(a0, b0, lo0, hi0);
-    mul128(a1, b1, lo1, hi1);
-    reduce128_lazy2(lo0, hi0, lo1, hi1, r0, r1);
+    uint64_t lo0, hi0, c0, lo1, hi1, c1;
+    mul128c(a0, b0, lo0, hi0, c0);
+    mul128c(a1, b1, lo1, hi1, c1);
+    reduce128c_lazy2(lo0, hi0, c0, lo1, hi1, c1, r0, r1);
 }
 
 // a * b, canonical (the lazy product's 7-instruction reduction plus the
 // 4-instruction canonical fold is shorter than reduce128's chains).
 __device__ __forceinline__ uint64_t mul(uint64_t a, uint64_t b) { return canon(mul_lazy(a, b)); }
+__device__ __forceinline__ uint64_t mul_nc(uint64_t a, uint64_t b) { return canon(mul_lazy_nc(a, b)); }
 
 __device__ __forceinline__ uint64_t sqr(uint64_t a) { return mul(a, a); }
 

@@ -626,7 +626,19 @@ __device__ __forceinline__ void strided2_tw_load(const NttPass &a, uint64_t tile
     }
 }
 
+// Butterfly product of the two-round kernels: the carry form (gl::mul) but
+// in the in-place 8-stage kernel, where its schedule exceeds the 128 VGPRs
+// of four waves per SIMD (20 VGPRs spilled, against 6 with gl::mul_nc; the
+// 64 x 2^20 forward NTT took 1.03 against 0.95 ms).
+template <bool NC>
+__device__ __forceinline__ uint64_t strided2_mul(uint64_t a, uint64_t b)
+{
+    if constexpr (NC) return gl::mul_nc(a, b);
+    else return gl::mul(a, b);
+}
+
 // First round: stages K+3..K on the registers (element i = top j-bits).
+template <bool NC>
 __device__ __forceinline__ void strided2_round1(uint64_t (&x)[kNttPerThread], const uint64_t (&wa)[15])
 {
 #pragma unroll
@@ -637,13 +649,13 @@ __device__ __forceinline__ void strided2_round1(uint64_t (&x)[kNttPerThread], co
             const int f = e | (1 << dd);
             const uint64_t y0 = x[e], y1 = x[f];
             x[e] = gl::add(y0, y1);
-            x[f] = gl::mul(gl::sub(y0, y1), wa[stage_off(4, dd) + (e & ((1 << dd) - 1))]);
+            x[f] = strided2_mul<NC>(gl::sub(y0, y1), wa[stage_off(4, dd) + (e & ((1 << dd) - 1))]);
         }
     }
 }
 
 // Second round: stages K-1..0 through LDS (ntt_round with b0 = 0, RB = c).
-template <int M>
+template <int M, bool NC>
 __device__ __forceinline__ void strided2_round2(uint64_t *lds, uint32_t t, const uint64_t (&wb)[strided2_nwb<M>()])
 {
     constexpr int K = M - 4, C = kNttTileLog - M, E = kNttPerThread;
@@ -664,7 +676,7 @@ __device__ __forceinline__ void strided2_round2(uint64_t *lds, uint32_t t, const
             const int f = e | (1 << dd);
             const uint64_t y0 = x[e], y1 = x[f];
             x[e] = gl::add(y0, y1);
-            x[f] = gl::mul(gl::sub(y0, y1), wb[stage_off(K, dd) + (e & ((1 << dd) - 1))]);
+            x[f] = strided2_mul<NC>(gl::sub(y0, y1), wb[stage_off(K, dd) + (e & ((1 << dd) - 1))]);
         }
     }
 #pragma unroll
@@ -735,11 +747,12 @@ k_ntt_strided2_tiles(const NttPass *__restrict__ passes, uint32_t npasses, NttPa
 #pragma unroll
         for (int i = 0; i < kNttPerThread; i++) x[i] = gl::mul(x[i], gmem(r.fac + strided2_step<M>(a, i))[off]);
     }
-    strided2_round1(x, wa);
+    constexpr bool NC = MODE == kSrcInPlace && M == 8;
+    strided2_round1<NC>(x, wa);
 #pragma unroll
     for (int i = 0; i < kNttPerThread; i++) lds[lds_slot(t + kNttThreads * i)] = x[i];
     __syncthreads();
-    strided2_round2<M>(lds, t, wb);
+    strided2_round2<M, NC>(lds, t, wb);
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < kNttPerThread; i++)
