@@ -50,19 +50,19 @@ HBM_PEAK_GBS = 8000.0
 #   permutations run their MDS layers on the matrix cores
 #   (v_mfma_i32_32x32x32_i8).
 #   keyed by (merkle_tile_log, merkle_valu)
-VALU_WAVE_INSTS_PER_PERM = {(8, 0): 220.24, (9, 0): 220.19, (10, 0): 220.19, (9, 1): 302.88}
+VALU_WAVE_INSTS_PER_PERM = {(8, 0): 205.49, (9, 0): 205.43, (10, 0): 205.43, (9, 1): 288.12}
 MFMA_WAVE_INSTS_PER_PERM = {(8, 0): 8.437, (9, 0): 8.437, (10, 0): 8.437, (9, 1): 0.0}
 #   SHA-256 (first 16 hex digits) of the leaf kernel's sources those counts
 #   were measured on (leaf_kernel_digest()); a build whose sources differ is
 #   flagged in the JSON ("counters_match_build": false)
-LEAF_COUNTERS_SOURCES = "436678e9dd45d2db"
+LEAF_COUNTERS_SOURCES = "de2226a6565c05d7"
 #   issue slots (one per 4 cycles) that one such MFMA takes from the vector
 #   issue of its SIMD beside full-rate VALU work: measured live by the
 #   "issue" leg (zkx_issue_probe); this value only when that leg is skipped
 MFMA_ISSUE_SLOTS = 3.25
 #   HBM bytes per commit: FETCH_SIZE x 2 (the counter reads half the bytes of
 #   a plain 1 GiB copy) + WRITE_SIZE x 1, KiB x 1024, by work-group size
-LEAF_TRAFFIC_BYTES_PER_COMMIT = {(8, 0): 7.723e8, (9, 0): 7.786e8, (10, 0): 8.055e8, (9, 1): 7.827e8}
+LEAF_TRAFFIC_BYTES_PER_COMMIT = {(8, 0): 7.636e8, (9, 0): 7.669e8, (10, 0): 8.049e8, (9, 1): 7.912e8}
 # Sources of the leaf-hash launch (zeknox_amd/csrc)
 LEAF_KERNEL_SOURCES = ("gl64.h", "gl_constants.h", "gmem.h", "hash.h", "poseidon_mfma.h", "merkle_kernels.h",
                        "merkle_launch.h", "merkle_launch_impl.h", "merkle_types.h", "merkle_poseidon.hip")
@@ -71,9 +71,9 @@ NTT_TRAFFIC_BYTES_PER_COMMIT = 8.855e8
 #   VALU wave-instructions of the NTT launches per commit (both passes,
 #   SQ_INSTS_VALU), keyed by the LDE's pass split (stages per pass), measured
 #   on the sources NTT_COUNTERS_SOURCES
-NTT_VALU_WAVE_INSTS_PER_COMMIT = {(3, 12): 1.478e8, (4, 11): 1.463e8, (5, 10): 1.560e8, (6, 9): 1.500e8,
-                                  (7, 8): 1.547e8, (8, 7): 1.530e8}
-NTT_COUNTERS_SOURCES = "5cd02a58bc4df9c8"
+NTT_VALU_WAVE_INSTS_PER_COMMIT = {(3, 12): 1.406e8, (4, 11): 1.391e8, (5, 10): 1.468e8, (6, 9): 1.428e8,
+                                  (7, 8): 1.460e8, (8, 7): 1.453e8}
+NTT_COUNTERS_SOURCES = "8bebc20fa8d74bff"
 # 32-bit VALU issue peak: 256 CUs x 4 SIMDs x one wave64 instruction per 4
 # cycles x 2.4 GHz = 39.3 T lane-ops/s (MI355X's published 78.6 TFLOP/s
 # vector FP64 counts an FMA as 2 flops on the same issue rate)
