@@ -222,6 +222,81 @@ __device__ __forceinline__ void poseidon_hash_two_pair(const uint64_t l[4], cons
     for (int i = 0; i < 4; i++) out[i] = gl::canon(x[i]);
 }
 
+// -------------------------------------------------- Poseidon on a lane row
+//
+// The narrowest levels of a tile (a few nodes per work-group) are bound by
+// one permutation's latency even on a lane pair.  Sixteen lanes of a wave
+// (a row: lanes 16r .. 16r + 15) hash one node: lane k < 12 holds state
+// element k, lanes 12..15 ride along on element k - 12 (never read by the
+// others).  MDS row k is sum_m CIRC[m] x_{(k + m) mod 12} (+ DIAG[k] x_k):
+// each lane fetches the eleven other elements in that rotated order with
+// ds_bpermute (two 32-bit words each), so every coefficient but the first is
+// the same immediate on all lanes.  All 16 lanes of a row must be active.
+
+__device__ __forceinline__ uint32_t row_fetch(uint32_t byte_addr, uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_ds_bpermute((int)byte_addr, (int)v);
+}
+
+// x <- row k of MDS * state + kc (kc: this lane's constant of the next
+// round); src[m] = 4 * (the lane holding element (k + m) mod 12); c0 =
+// CIRC[0] + DIAG[k].  Lazy in and out, as poseidon_mds.
+__device__ __forceinline__ uint64_t poseidon_mds_row(uint64_t x, const uint32_t (&src)[12], uint64_t kc, uint32_t c0)
+{
+    uint32_t lo[12], hi[12];
+    lo[0] = gl::lo32(x);
+    hi[0] = gl::hi32(x);
+#pragma unroll
+    for (int m = 1; m < 12; m++) {
+        lo[m] = row_fetch(src[m], lo[0]);
+        hi[m] = row_fetch(src[m], hi[0]);
+    }
+    uint64_t A = gl::vopaque64((uint64_t)lo[0] * c0 + gl::lo32(kc));
+    uint64_t B = gl::vopaque64((uint64_t)hi[0] * c0 + gl::hi32(kc));
+#pragma unroll
+    for (int m = 1; m < 12; m++) {
+        const uint32_t c = gl::opaque((uint32_t)ZK_POSEIDON_MDS_CIRC[m]);
+        A = gl::vopaque64((uint64_t)lo[m] * c + A);
+        B = gl::vopaque64((uint64_t)hi[m] * c + B);
+    }
+    return limb_combine(A, B);
+}
+
+// poseidon_permute with the state spread over a row (x: this lane's element,
+// k = lane mod 16).  Lazy in and out.
+__device__ __forceinline__ uint64_t poseidon_permute_row(uint64_t x, uint32_t k)
+{
+    const uint32_t kk = k < 12 ? k : k - 12;
+    const uint32_t base = (threadIdx.x & 63u) & ~15u;  // first lane of the row within the wave
+    uint32_t src[12];
+#pragma unroll
+    for (int m = 0; m < 12; m++) {
+        const uint32_t e = kk + (uint32_t)m;
+        src[m] = 4u * (base + (e >= 12 ? e - 12 : e));
+    }
+    uint32_t c0 = (uint32_t)ZK_POSEIDON_MDS_CIRC[0];
+#pragma unroll
+    for (int i = 0; i < 12; i++)
+        if (ZK_POSEIDON_MDS_DIAG[i] != 0 && kk == (uint32_t)i) c0 += (uint32_t)ZK_POSEIDON_MDS_DIAG[i];
+    x = gl::add(x, c_poseidon_rc[kk]);
+    // rounds 0-3 and 26-29 full, 4-25 partial (the S-box on element 0 only)
+#pragma unroll 1
+    for (int r = 0; r < 30; r++) {
+        const uint64_t s = gl::pow7_lazy(x);
+        x = (r < 4 || r >= 26 || kk == 0) ? s : x;
+        x = poseidon_mds_row(x, src, c_poseidon_rc[12 * (r + 1) + kk], c0);
+    }
+    return x;
+}
+
+// hash_two<kPoseidon>(l, r) on a row: state [l, r, 0]; lane k < 4 returns
+// element k of the digest (canonical).
+__device__ __forceinline__ uint64_t poseidon_hash_two_row(const uint64_t l[4], const uint64_t r[4], uint32_t k)
+{
+    const uint64_t x = k < 4 ? l[k] : k < 8 ? r[k - 4] : 0;
+    return gl::canon(poseidon_permute_row(x, k));
+}
+
 // --------------------------------------------------------------- Poseidon2
 
 // External layer (lazy in and out) M_E = [[2M4, M4, M4], [M4, 2M4, M4], [M4, M4, 2M4]] with

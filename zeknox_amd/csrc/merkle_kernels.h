@@ -215,10 +215,14 @@ k_merkle_tile(const MerkleArgs *__restrict__ jobs, uint32_t njobs, MerkleArgs on
         }
         // Nodes of a level: with PAIR (Poseidon, Poseidon2) lanes 2n and 2n+1
         // hash node n together (a level never has more than TPB / 2 nodes)
-        // and lane 2n writes the digest; otherwise lane n hashes node n.
+        // and lane 2n writes the digest; a Poseidon level of at most TPB / 16
+        // nodes runs on rows (ROW: lanes 16n .. 16n + 15, lane 16n + k
+        // writes digest element k < 4); otherwise lane n hashes node n.
         constexpr bool PAIR = HT == kPoseidon || HT == kPoseidon2;
-        const uint32_t n = PAIR ? t >> 1 : t;
-        const bool writer = PAIR ? (t & 1) == 0 : true;
+        const bool ROW = HT == kPoseidon && (cur >> 1) * 16 <= TPB;
+        const uint32_t k = t & 15;
+        const uint32_t n = ROW ? t >> 4 : PAIR ? t >> 1 : t;
+        const bool writer = ROW ? k < 4 : PAIR ? (t & 1) == 0 : true;
         auto hash_node = [&](uint64_t out[4]) {
             if constexpr (HT == kPoseidon) poseidon_hash_two_pair(lds[2 * n], lds[2 * n + 1], out, (t & 1) != 0);
             else if constexpr (HT == kPoseidon2) poseidon2_hash_two_pair(lds[2 * n], lds[2 * n + 1], out, (t & 1) != 0);
@@ -228,10 +232,17 @@ k_merkle_tile(const MerkleArgs *__restrict__ jobs, uint32_t njobs, MerkleArgs on
             // width-2 top level (cur >= 2 here): caps of the subtrees in this tile
             const uint32_t pairs = cur >> 1;
             if (n < pairs) {
+                const uint64_t sub = (first >> 1) + n;
+                if constexpr (HT == kPoseidon) {
+                    if (ROW) {
+                        const uint64_t v = poseidon_hash_two_row(lds[2 * n], lds[2 * n + 1], k);
+                        if (writer) gmem(a.cap)[4 * sub + k] = v;
+                        return;
+                    }
+                }
                 uint64_t out[4];
                 hash_node(out);
                 if (writer) {
-                    const uint64_t sub = (first >> 1) + n;
 #pragma unroll
                     for (int i = 0; i < 4; i++) gmem(a.cap)[4 * sub + i] = out[i];
                 }
@@ -240,7 +251,15 @@ k_merkle_tile(const MerkleArgs *__restrict__ jobs, uint32_t njobs, MerkleArgs on
         }
         const uint32_t half = cur >> 1;
         uint64_t out[4];
-        if (n < half) hash_node(out);
+        uint64_t v = 0;  // ROW: this lane's digest element
+        if (n < half) {
+            if constexpr (HT == kPoseidon) {
+                if (ROW) v = poseidon_hash_two_row(lds[2 * n], lds[2 * n + 1], k);
+                else hash_node(out);
+            } else {
+                hash_node(out);
+            }
+        }
         __syncthreads();
         lg_w -= 1;
         first >>= 1;
@@ -248,10 +267,15 @@ k_merkle_tile(const MerkleArgs *__restrict__ jobs, uint32_t njobs, MerkleArgs on
             const uint64_t g = first + n;
             const uint64_t sub = g >> lg_w, idx = g & ((1ull << lg_w) - 1);
             const gptr<uint64_t> dst = node_ptr(a, sub, lg_w, idx);
+            if (ROW) {
+                dst[k] = v;
+                lds[n][k] = v;
+            } else {
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                dst[i] = out[i];
-                lds[n][i] = out[i];
+                for (int i = 0; i < 4; i++) {
+                    dst[i] = out[i];
+                    lds[n][i] = out[i];
+                }
             }
         }
         cur = half;
