@@ -107,9 +107,9 @@ def parse():
                          "2^18 leaves), always fused into the leaf launch, or always one launch per level")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="served-stream duration (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=200.0,
-                    help="served-stream arrivals per second (default: about 80% of the highest rate the server "
-                         "keeps up with, 240-260/s in the sweeps)")
+    ap.add_argument("--serve-rate", type=float, default=224.0,
+                    help="served-stream arrivals per second (default: 80% of the highest rate the server keeps "
+                         "up with, 280/s in the sweep)")
     ap.add_argument("--serve-streams", type=int, default=6,
                     help="commit engines (streams) of the server (6: lowest latencies at 200/s, against 3 and 8)")
     ap.add_argument("--serve-trace", default=None,
