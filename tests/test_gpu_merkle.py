@@ -122,14 +122,41 @@ def test_invalid_arguments_are_reported(zk):
     assert np.all(c.to_host() == 0)  # nothing written
 
 
-def test_four_pass_tree_in_one_launch(zk):
-    """2^25 leaves, one cap, 256-leaf tiles: the leaf pass and three upper
-    passes run in one launch (the last work-group of each group of tiles goes
-    on).  Keccak with 2-element leaves keeps the oracle's work to the 2^25
-    inner nodes."""
-    print("four-pass tree 2^25 x 2, cap height 0, Keccak", flush=True)
+@pytest.fixture
+def lone_levels(zk, request):
+    """Merkle schedule of one tree through the C ABI: finished inside the
+    leaf launch ("fused") or one launch per level above the leaves
+    ("by_level"); restored to the library default afterwards."""
+    zk.set_lone_level_launches(None if request.param == "fused" else 0)
+    yield request.param
+    zk.set_lone_level_launches(zk.LONE_LEVEL_LAUNCHES_DEFAULT)
+
+
+@pytest.mark.parametrize("lone_levels", ["fused", "by_level"], indirect=True)
+def test_four_pass_tree(zk, lone_levels):
+    """2^25 leaves, one cap, 256-leaf tiles: fused, the leaf pass and three
+    upper passes run in one launch (the last work-group of each group of
+    tiles goes on); by level, 25 level launches.  Keccak with 2-element
+    leaves keeps the oracle's work to the 2^25 inner nodes."""
+    print(f"four-pass tree 2^25 x 2, cap height 0, Keccak, {lone_levels}", flush=True)
     leaves = ob.canonical(2 << 25, 2525)
     dig, caps = gpu_tree(zk, leaves, 2, 0, 1)
     edig, ecap = ob.merkle(leaves, 2, 0, 1)
+    assert np.array_equal(caps, ecap)
+    assert np.array_equal(dig, edig)
+
+
+@pytest.mark.parametrize("lone_levels", ["by_level"], indirect=True)
+@pytest.mark.parametrize("ht", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("lg_l,ls,h", [(1, 7, 0), (2, 9, 1), (5, 4, 3), (7, 6, 7), (9, 12, 9), (10, 135, 4),
+                                       (12, 80, 4), (13, 135, 0)])
+def test_tree_by_level_vs_oracle(zk, lone_levels, ht, lg_l, ls, h):
+    """One tree with its levels launched apart (leaf digests, then one launch
+    per level down to the caps): caps == leaves, one-leaf subtrees, width-2
+    tops, one and many subtrees, every hash type."""
+    print(f"tree by level lg_l={lg_l} ls={ls} h={h} ht={ht}", flush=True)
+    leaves = ob.canonical(ls << lg_l, 3000 * lg_l + 10 * ls + h + ht)
+    dig, caps = gpu_tree(zk, leaves, ls, h, ht)
+    edig, ecap = ob.merkle(leaves, ls, h, ht)
     assert np.array_equal(caps, ecap)
     assert np.array_equal(dig, edig)

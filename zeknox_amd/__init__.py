@@ -158,6 +158,7 @@ def _load():
         "zkx_set_autotune": (RustError, [sz, ctypes.c_int]),
         "zkx_set_commit_graphs": (RustError, [sz, ctypes.c_int]),
         "zkx_set_level_launches": (RustError, [sz, u64]),
+        "zkx_set_lone_level_launches": (RustError, [sz, u64]),
         "zkx_commit_graphs_captured": (sz, [sz]),
         "zkx_tuning_clear": (RustError, [sz]),
         "zkx_tuning_save": (RustError, [sz, ctypes.c_char_p]),
@@ -475,6 +476,16 @@ def set_level_launches(min_leaves, device_id=0):
     levels with one launch per level (0: always, None: never)."""
     v = (1 << 64) - 1 if min_leaves is None else int(min_leaves)
     _check(lib.zkx_set_level_launches(device_id, v))
+
+
+LONE_LEVEL_LAUNCHES_DEFAULT = 1 << 20
+
+
+def set_lone_level_launches(min_leaves, device_id=0):
+    """One tree through the C-ABI Merkle entry points builds its levels with
+    one launch per level from min_leaves leaves (0: always, None: never)."""
+    v = (1 << 64) - 1 if min_leaves is None else int(min_leaves)
+    _check(lib.zkx_set_lone_level_launches(device_id, v))
 
 
 def commit_graphs_captured(device_id=0):

@@ -262,12 +262,7 @@ void Engine::commit(const CommitJob *jobs, size_t n, void *scratch, const Tuning
                 uint64_t blocks = 0;
                 for (size_t i = 0; i < n; i++) {
                     if ((int)jobs[i].hash_type != ht || mk[i].lg_w < k) continue;
-                    MerkleArgs a{};
-                    a.dig = mk[i].dig;
-                    a.cap = mk[i].cap;
-                    a.sdl = mk[i].sdl;
-                    a.lg_w = mk[i].lg_w - k;  // the level written (0: the caps)
-                    a.count = (mk[i].count >> mk[i].lg_w) << a.lg_w;
+                    MerkleArgs a = merkle_level_args(mk[i], mk[i].lg_w - k);  // the level written (0: the caps)
                     a.tile_begin = blocks;
                     blocks += (a.count + kMerkleLevelLanes - 1) / kMerkleLevelLanes;
                     list.push_back(a);

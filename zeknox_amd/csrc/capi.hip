@@ -475,6 +475,11 @@ RustError zkx_set_level_launches(size_t device_id, uint64_t min_leaves)
     return guarded([&] { device(device_id).level_launch_leaves = min_leaves; });
 }
 
+RustError zkx_set_lone_level_launches(size_t device_id, uint64_t min_leaves)
+{
+    return guarded([&] { device(device_id).lone_level_leaves = min_leaves; });
+}
+
 size_t zkx_commit_graphs_captured(size_t device_id)
 {
     try {

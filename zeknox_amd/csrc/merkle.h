@@ -44,7 +44,11 @@ void launch_merkle_pass(hipStream_t stream, int hash_type, const MerkleArgs *dev
 // level descriptors in device memory; `blocks` work-groups of 256 lanes in
 // total, issued in ranges of at most `chunk` work-groups when chunk > 0.
 void launch_merkle_level(hipStream_t stream, int hash_type, const MerkleArgs *dev_list, uint32_t n, uint64_t blocks,
-                         uint64_t chunk = 0);
+                         uint64_t chunk = 0, const MerkleArgs &one = MerkleArgs{});
+// Level descriptor (k_merkle_level) of the level of width 2^lg_w per subtree
+// (0: the caps) of the tree whose leaf pass is `leaf` (plan_merkle's first
+// pass); tile_begin 0.
+MerkleArgs merkle_level_args(const MerkleArgs &leaf, uint32_t lg_w);
 // The whole tree in one launch on the device stream; call with d.mu held.
 void merkle_build(Device &d, const MerkleSpec &s);
 

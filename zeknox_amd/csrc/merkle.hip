@@ -122,8 +122,19 @@ void launch_merkle_pass(hipStream_t stream, int hash_type, const MerkleArgs *dev
     }
 }
 
+MerkleArgs merkle_level_args(const MerkleArgs &leaf, uint32_t lg_w)
+{
+    MerkleArgs a{};
+    a.dig = leaf.dig;
+    a.cap = leaf.cap;
+    a.sdl = leaf.sdl;
+    a.lg_w = lg_w;
+    a.count = (leaf.count >> leaf.lg_w) << lg_w;
+    return a;
+}
+
 void launch_merkle_level(hipStream_t stream, int hash_type, const MerkleArgs *dev_list, uint32_t n, uint64_t blocks,
-                         uint64_t chunk)
+                         uint64_t chunk, const MerkleArgs &one)
 {
     if (blocks == 0) return;
     if (blocks > 0x7FFFFFFFull) invalid("tree level too large for one launch");
@@ -132,11 +143,13 @@ void launch_merkle_level(hipStream_t stream, int hash_type, const MerkleArgs *de
     for (uint64_t b0 = 0; b0 < blocks; b0 += step) {
         const uint64_t nb = std::min(step, blocks - b0);
         switch (hash_type) {
-        case kPoseidon2: launch_merkle_level_hash<kPoseidon2>(stream, dev_list, n, nb, (uint32_t)b0); break;
-        case kKeccak: launch_merkle_level_hash<kKeccak>(stream, dev_list, n, nb, (uint32_t)b0); break;
-        case kMonolith: launch_merkle_level_hash<kMonolith>(stream, dev_list, n, nb, (uint32_t)b0); break;
-        case kPoseidonBN128: launch_merkle_level_hash<kPoseidonBN128>(stream, dev_list, n, nb, (uint32_t)b0); break;
-        default: launch_merkle_level_hash<kPoseidon>(stream, dev_list, n, nb, (uint32_t)b0); break;
+        case kPoseidon2: launch_merkle_level_hash<kPoseidon2>(stream, dev_list, n, one, nb, (uint32_t)b0); break;
+        case kKeccak: launch_merkle_level_hash<kKeccak>(stream, dev_list, n, one, nb, (uint32_t)b0); break;
+        case kMonolith: launch_merkle_level_hash<kMonolith>(stream, dev_list, n, one, nb, (uint32_t)b0); break;
+        case kPoseidonBN128:
+            launch_merkle_level_hash<kPoseidonBN128>(stream, dev_list, n, one, nb, (uint32_t)b0);
+            break;
+        default: launch_merkle_level_hash<kPoseidon>(stream, dev_list, n, one, nb, (uint32_t)b0); break;
         }
         ZKX_HIP(hipGetLastError());
     }
@@ -151,6 +164,19 @@ void merkle_build(Device &d, const MerkleSpec &s)
     sv.leaf_valu = tun.merkle_valu;
     MerkleArgs passes[kMerkleMaxPasses];
     const uint32_t np = plan_merkle(sv, passes, tl);
+    if (s.num_caps < s.num_leaves && lone_tree_levels_apart(s.num_leaves, s.hash_type, d.lone_level_leaves.load())) {
+        // a large tree: the leaf digests in one launch, then one launch per
+        // level (k_merkle_level), each level's descriptor passed by value
+        MerkleArgs leaf = passes[0];
+        leaf.leaves_only = 1;
+        launch_merkle_pass(d.stream, (int)s.hash_type, nullptr, 0, leaf, leaf.count >> leaf.tile_log, tl);
+        for (uint32_t w = leaf.lg_w; w-- > 0;) {
+            const MerkleArgs lv = merkle_level_args(leaf, w);
+            launch_merkle_level(d.stream, (int)s.hash_type, nullptr, 0,
+                                (lv.count + kMerkleLevelLanes - 1) / kMerkleLevelLanes, 0, lv);
+        }
+        return;
+    }
     MerkleArgs f;
     const uint64_t nctr = fuse_merkle_passes(passes, np, f);
     f.tile_begin = 0;

@@ -295,14 +295,14 @@ k_merkle_tile(const MerkleArgs *__restrict__ jobs, uint32_t njobs, MerkleArgs on
 // hash_two of the 8 consecutive words of its two children at level lg_w + 1.
 // Poseidon hashes 64 nodes per wave on the matrix cores (the state [l, r, 0]
 // laid out as leaf_hash_mfma lays out a leaf's first chunk), the other hashes
-// one node per lane.
+// one node per lane.  njobs == 0: the single descriptor `one` (a lone tree).
 template <int HT>
 __global__ void __launch_bounds__(kMerkleLevelLanes) __attribute__((amdgpu_waves_per_eu(merkle_waves_per_eu(HT))))
-k_merkle_level(const MerkleArgs *__restrict__ jobs, uint32_t njobs, uint32_t block0)
+k_merkle_level(const MerkleArgs *__restrict__ jobs, uint32_t njobs, MerkleArgs one, uint32_t block0)
 {
     __shared__ uint64_t kt[HT == kPoseidon ? kMfmaRcWords : 1];
     const uint32_t bid = blockIdx.x + block0;
-    const MerkleArgs a = *find_job(jobs, njobs, bid);
+    const MerkleArgs a = njobs ? *find_job(jobs, njobs, bid) : one;
     const uint32_t t = threadIdx.x, lw = a.lg_w;
     const uint64_t g0 = (uint64_t)(bid - a.tile_begin) * kMerkleLevelLanes;
     const uint64_t wmask = (1ull << lw) - 1;

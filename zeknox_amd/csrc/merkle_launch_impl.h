@@ -19,11 +19,11 @@ void launch_merkle_hash(hipStream_t stream, const MerkleArgs *dev_list, uint32_t
 }
 
 template <int HT>
-void launch_merkle_level_hash(hipStream_t stream, const MerkleArgs *dev_list, uint32_t n, uint64_t blocks,
-                              uint32_t block0)
+void launch_merkle_level_hash(hipStream_t stream, const MerkleArgs *dev_list, uint32_t n, const MerkleArgs &one,
+                              uint64_t blocks, uint32_t block0)
 {
     hipLaunchKernelGGL((k_merkle_level<HT>), dim3((uint32_t)blocks), dim3(kMerkleLevelLanes), 0, stream, dev_list, n,
-                       block0);
+                       one, block0);
 }
 
 }  // namespace zkx

@@ -5,6 +5,7 @@ namespace zkx {
 
 template void launch_merkle_hash<kPoseidon>(hipStream_t, const MerkleArgs *, uint32_t, const MerkleArgs &, uint64_t,
                                      uint32_t, uint32_t);
-template void launch_merkle_level_hash<kPoseidon>(hipStream_t, const MerkleArgs *, uint32_t, uint64_t, uint32_t);
+template void launch_merkle_level_hash<kPoseidon>(hipStream_t, const MerkleArgs *, uint32_t, const MerkleArgs &,
+                                         uint64_t, uint32_t);
 
 }  // namespace zkx

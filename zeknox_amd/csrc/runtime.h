@@ -117,6 +117,12 @@ struct Device {
     // leaves up, 1.14-1.16 below (each level launch costs at least one
     // permutation's latency, about 43 us).
     std::atomic<uint64_t> level_launch_leaves{1ull << 18};
+    // The same for one tree through the C-ABI Merkle entry points
+    // (fill_digests_buf_linear_gpu_with_gpu_ptr; the allocation-free entry
+    // point always runs one fused launch), Poseidon trees from twice this
+    // (lone_tree_levels_apart): a lone tree's narrow levels are
+    // latency-bound either way, so only wide trees gain.
+    std::atomic<uint64_t> lone_level_leaves{1ull << 20};
     Tuning tuning_default();
     // The schedule of the C-ABI Merkle entry points for one tree: the
     // default, with lone_tree_tile_log()'s work-groups unless the caller set

@@ -53,4 +53,18 @@ using TuneKey = std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, bool>;
 
 bool valid_tuning(const Tuning &t);
 
+// Whether one tree through the C-ABI Merkle entry points builds its levels
+// with one launch per level (after a leaf-digests-only launch) rather than
+// inside its leaf launch, given the device threshold `min_leaves`.
+// Measured (fused against by level, 135-element leaves, cap 4, one GPU):
+// Poseidon 2^18 2.25 / 2.40, 2^19 4.38 / 4.41, 2^20 8.14 / 8.16, 2^21 16.09 /
+// 15.44 ms; Poseidon2 2^18 2.26 / 2.38, 2^19 4.07 / 4.00, 2^20 7.47 / 7.16,
+// 2^21 14.38 / 13.41 ms.  Poseidon's fused levels (rows of 16 lanes, lane
+// pairs) are the faster, so its trees switch at twice the threshold.
+inline bool lone_tree_levels_apart(uint64_t num_leaves, uint64_t hash_type, uint64_t min_leaves)
+{
+    const uint64_t t = hash_type == 0 ? (min_leaves > (~0ull >> 1) ? ~0ull : 2 * min_leaves) : min_leaves;
+    return num_leaves >= t;
+}
+
 }  // namespace zkx
