@@ -357,20 +357,23 @@ __device__ __forceinline__ uint64_t sum12(const uint64_t s[12])
 }
 
 // Internal layer: s_i <- s_i * d_i + sum(s) (d = MATRIX_DIAG_12 = diag - 1),
-// each as one 128-bit value reduced once (lazy in and out).
+// each as one 128-bit value reduced once (lazy in and out).  The product's
+// middle carry (gl::mul128c) enters the reduction as a borrow: the sum added
+// to the low word carries at most 1 into the high word, and the total stays
+// below (2^64 - 1)^2 + 2^64, so the high word plus the carry fits 64 bits.
 __device__ __forceinline__ void p2_internal(uint64_t s[12])
 {
     const uint64_t sum = sum12(s);
 #pragma unroll
     for (int i = 0; i < 12; i++) {
-        uint64_t lo, hi;
-        gl::mul128(s[i], ZK_POSEIDON2_DIAG[i], lo, hi);
+        uint64_t lo, hi, pc;
+        gl::mul128c(s[i], ZK_POSEIDON2_DIAG[i], lo, hi, pc);
         uint32_t c1, c2, c3, c4;
         const uint32_t l0 = __builtin_addc(gl::lo32(lo), gl::lo32(sum), 0u, &c1);
         const uint32_t l1 = __builtin_addc(gl::hi32(lo), gl::hi32(sum), c1, &c2);
         const uint32_t h0 = __builtin_addc(gl::lo32(hi), 0u, c2, &c3);
         const uint32_t h1 = __builtin_addc(gl::hi32(hi), 0u, c3, &c4);
-        s[i] = gl::reduce128_lazy(gl::pack(l0, l1), gl::pack(h0, h1));
+        s[i] = gl::reduce128c_lazy(gl::pack(l0, l1), gl::pack(h0, h1), pc);
     }
 }
 
@@ -471,14 +474,14 @@ __device__ __forceinline__ void poseidon2_permute_pair(uint64_t s[12], bool h)
         const uint64_t sum = gl::reduce96_lazy(gl::pack(lo, hi), top);
 #pragma unroll
         for (int m = 0; m < 6; m++) {
-            uint64_t plo, phi;
-            gl::mul128(own[m], d[m], plo, phi);
+            uint64_t plo, phi, pc;
+            gl::mul128c(own[m], d[m], plo, phi, pc);
             uint32_t c1, c2, c3, c4;
             const uint32_t l0 = __builtin_addc(gl::lo32(plo), gl::lo32(sum), 0u, &c1);
             const uint32_t l1 = __builtin_addc(gl::hi32(plo), gl::hi32(sum), c1, &c2);
             const uint32_t h0 = __builtin_addc(gl::lo32(phi), 0u, c2, &c3);
             const uint32_t h1 = __builtin_addc(gl::hi32(phi), 0u, c3, &c4);
-            own[m] = gl::reduce128_lazy(gl::pack(l0, l1), gl::pack(h0, h1));
+            own[m] = gl::reduce128c_lazy(gl::pack(l0, l1), gl::pack(h0, h1), pc);
         }
     }
     merge(own);
