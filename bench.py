@@ -55,7 +55,7 @@ MFMA_WAVE_INSTS_PER_PERM = {(8, 0): 8.437, (9, 0): 8.437, (10, 0): 8.437, (9, 1)
 #   SHA-256 (first 16 hex digits) of the leaf kernel's sources those counts
 #   were measured on (leaf_kernel_digest()); a build whose sources differ is
 #   flagged in the JSON ("counters_match_build": false)
-LEAF_COUNTERS_SOURCES = "d267d613aca84925"
+LEAF_COUNTERS_SOURCES = "62b726b6c1fc2972"
 #   issue slots (one per 4 cycles) that one such MFMA takes from the vector
 #   issue of its SIMD beside full-rate VALU work: measured live by the
 #   "issue" leg (zkx_issue_probe); this value only when that leg is skipped
