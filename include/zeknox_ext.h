@@ -174,10 +174,11 @@ RustError zkx_set_commit_graphs(size_t device_id, int on);
  * setting changes a result. */
 RustError zkx_set_level_launches(size_t device_id, uint64_t min_leaves);
 /* The same for one tree through fill_digests_buf_linear_gpu_with_gpu_ptr and
- * the other C-ABI Merkle entry points: trees of at least min_leaves leaves
- * (default 2^20), Poseidon trees of at least twice that (their fused levels
- * are faster); the allocation-free entry point always finishes the tree in
- * its leaf launch. */
+ * the other C-ABI Merkle entry points: Poseidon and Poseidon2 trees of at
+ * least min_leaves leaves (default 2^20), Keccak-256 and Monolith trees of at
+ * least min_leaves / 4, Poseidon-BN128 trees of at least min_leaves / 32; the
+ * allocation-free entry point always finishes the tree in its leaf launch.
+ * 0: always by level; UINT64_MAX: never.  No setting changes a result. */
 RustError zkx_set_lone_level_launches(size_t device_id, uint64_t min_leaves);
 /* Graphs the device's zkx_commit_batch engine holds (tests, benchmarks). */
 size_t zkx_commit_graphs_captured(size_t device_id);

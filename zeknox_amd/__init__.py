@@ -483,7 +483,9 @@ LONE_LEVEL_LAUNCHES_DEFAULT = 1 << 20
 
 def set_lone_level_launches(min_leaves, device_id=0):
     """One tree through the C-ABI Merkle entry points builds its levels with
-    one launch per level from min_leaves leaves (0: always, None: never)."""
+    one launch per level from min_leaves leaves for Poseidon and Poseidon2,
+    min_leaves / 4 for Keccak-256 and Monolith, min_leaves / 32 for
+    Poseidon-BN128 (0: always, None: never)."""
     v = (1 << 64) - 1 if min_leaves is None else int(min_leaves)
     _check(lib.zkx_set_lone_level_launches(device_id, v))
 

@@ -119,7 +119,8 @@ struct Device {
     std::atomic<uint64_t> level_launch_leaves{1ull << 18};
     // The same for one tree through the C-ABI Merkle entry points
     // (fill_digests_buf_linear_gpu_with_gpu_ptr; the allocation-free entry
-    // point always runs one fused launch), Poseidon trees from twice this
+    // point always runs one fused launch); Keccak-256 and Monolith trees
+    // from a quarter of this, Poseidon-BN128 trees from a 32nd
     // (lone_tree_levels_apart): a lone tree's narrow levels are
     // latency-bound either way, so only wide trees gain.
     std::atomic<uint64_t> lone_level_leaves{1ull << 20};

@@ -55,16 +55,25 @@ bool valid_tuning(const Tuning &t);
 
 // Whether one tree through the C-ABI Merkle entry points builds its levels
 // with one launch per level (after a leaf-digests-only launch) rather than
-// inside its leaf launch, given the device threshold `min_leaves`.
-// Measured (fused against by level, 135-element leaves, cap 4, one GPU):
-// Poseidon 2^18 2.25 / 2.40, 2^19 4.38 / 4.41, 2^20 8.14 / 8.16, 2^21 16.09 /
-// 15.44 ms; Poseidon2 2^18 2.26 / 2.38, 2^19 4.07 / 4.00, 2^20 7.47 / 7.16,
-// 2^21 14.38 / 13.41 ms.  Poseidon's fused levels (rows of 16 lanes, lane
-// pairs) are the faster, so its trees switch at twice the threshold.
+// inside its leaf launch, given the device threshold `min_leaves` (default
+// 2^20): Poseidon and Poseidon2 trees switch at min_leaves, Keccak-256 and
+// Monolith at min_leaves / 4, Poseidon-BN128 at min_leaves / 32 (0: always).
+// Measured, fused against by level, 135-element leaves, cap 4, one GPU, ms:
+//   leaves  Poseidon      Poseidon2     Keccak-256     Monolith       Poseidon-BN128
+//   2^15    0.90 / 1.10   1.10 / 1.27   0.248 / 0.251  0.540 / 0.533  36.2 / 33.5
+//   2^17    1.30 / 1.48   1.43 / 1.59   0.330 / 0.333  0.695 / 0.697  48.5 / 46.2
+//   2^18    2.32 / 2.45   2.34 / 2.42   0.503 / 0.485  1.078 / 1.056  75.3 / 72.8
+//   2^19    4.33 / 4.34   4.02 / 4.01   1.095 / 0.779  2.069 / 1.760
+//   2^20    8.28 / 8.16   7.34 / 7.12   1.928 / 1.344  3.639 / 3.143
+// Poseidon's fused levels (rows of 16 lanes) are the fastest of the hashes';
+// Keccak's and Monolith's permutations are short, so their fused trees'
+// tops, one permutation's latency per level on ever fewer lanes, weigh
+// more; Poseidon-BN128's levels are slow enough that a full-GPU launch per
+// level pays at every size measured.
 inline bool lone_tree_levels_apart(uint64_t num_leaves, uint64_t hash_type, uint64_t min_leaves)
 {
-    const uint64_t t = hash_type == 0 ? (min_leaves > (~0ull >> 1) ? ~0ull : 2 * min_leaves) : min_leaves;
-    return num_leaves >= t;
+    const uint32_t shift = hash_type == 1 || hash_type == 4 ? 2 : hash_type == 2 ? 5 : 0;
+    return num_leaves >= (min_leaves >> shift);
 }
 
 }  // namespace zkx
