@@ -160,3 +160,25 @@ def test_tree_by_level_vs_oracle(zk, lone_levels, ht, lg_l, ls, h):
     edig, ecap = ob.merkle(leaves, ls, h, ht)
     assert np.array_equal(caps, ecap)
     assert np.array_equal(dig, edig)
+
+
+@pytest.mark.parametrize("ht,lg_l", [(2, 15), (1, 18), (4, 18), (0, 20), (3, 20)])
+def test_lone_schedules_agree_at_switch_sizes(zk, ht, lg_l):
+    """At the smallest tree each hash builds by level under the default rule
+    (Poseidon-BN128 2^15 leaves, Keccak-256 and Monolith 2^18, Poseidon and
+    Poseidon2 2^20), the default, fused and by-level schedules give the same
+    digests and caps (the fused schedule is the one checked against the
+    oracle at these hashes' smaller sizes)."""
+    print(f"lone schedules agree ht={ht} lg_l={lg_l}", flush=True)
+    ls, h = 20, 4
+    leaves = ob.canonical(ls << lg_l, 7100 + 37 * ht + lg_l)
+    out = {}
+    try:
+        for name, v in (("default", zk.LONE_LEVEL_LAUNCHES_DEFAULT), ("fused", None), ("by_level", 0)):
+            zk.set_lone_level_launches(v)
+            out[name] = gpu_tree(zk, leaves, ls, h, ht)
+    finally:
+        zk.set_lone_level_launches(zk.LONE_LEVEL_LAUNCHES_DEFAULT)
+    for name in ("fused", "by_level"):
+        assert np.array_equal(out[name][1], out["default"][1]), name
+        assert np.array_equal(out[name][0], out["default"][0]), name
